@@ -1,0 +1,186 @@
+"""Benchmark: templates/s of the BRP4 per-template search at N = 2^22 on one MI355X.
+
+Workload: the seeded N = 2^22 series (noise + injected binary pulsar) searched against a
+512-template bank (bank semantics, f0 400 Hz, false alarm 0.08, window 1000).  A step is
+one pass of the whole bank: resample -> FFT -> power -> harmonic sums -> candidates ->
+toplist, in the fused engine's batches.  The timed region starts with the series
+already resident in HBM.
+
+Also measured in the same run:
+  * the plain engine (one template per launch sequence, one kernel per stage) over the
+    same bank, and its toplist compared with the batched one (must be identical);
+  * per-kernel device times from HIP events on the launch stream over the timed region,
+    giving the roofline of the dominant kernel;
+  * the CPU baseline: the oracle's FP32 CPU pipeline on a bounded sample of the same
+    templates, on the host's cores.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+for _p in (ROOT / "boinc-einstein-bpr4-v3d_amd", ROOT / "tests"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--templates", type=int, default=512)
+    ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--batch", type=int, default=64, help="templates per run (toplist thresholds frozen per run)")
+    ap.add_argument("--group", type=int, default=8, help="templates per kernel launch")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--plain-templates", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the bank is design-only",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+
+    import brp_hip as brp
+    import workloads
+
+    t0 = time.time()
+    w = workloads.bank(count=args.templates, n=args.n)
+    n, U = w.n, w.n
+    H = w.templates[0].harmonic
+    log(f"[bench] workload {w.name}, spin {w.spin_hz:.3f} Hz, built in {time.time() - t0:.1f} s")
+
+    # ---------------------------------------------------------------- batched --
+    g = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
+                graph=not args.no_graph, group=args.group)
+    log("[bench]", g.device_info())
+
+    def bank_pass():
+        top = brp.Toplist(100)
+        _, counts, gpu_ms = g.bank_search(w.orbits, U, workloads.SAMPLE_US, top)
+        return top, counts, gpu_ms
+
+    for i in range(args.warmup):
+        bank_pass()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    kernel_total = np.zeros(4)
+    launches = 0
+    t_start = time.perf_counter()
+    gpu_ms_total = 0.0
+    for i in range(args.steps):
+        top, counts, gpu_ms = bank_pass()
+        gpu_ms_total += gpu_ms
+        log(f"[bench] step {i + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t_start
+    # Per-kernel event times of the last run of the last step, and launches per run.
+    km = np.array(g.kernel_ms())
+    per_run = min(args.batch, args.templates)
+    groups_per_run = -(-per_run // args.group)
+    templates_per_s = args.templates * args.steps / elapsed
+    batched_rows = [(r.bin, r.harmonics, r.period, r.power) for r in top.table()]
+    final_bins = [r.bin for r in top.final()]
+    g.close()
+    log(f"[bench] batched: {templates_per_s:.1f} templates/s; kernel ms of last run {km.tolist()}")
+
+    # ------------------------------------------------------------------ plain --
+    plain_n = min(args.plain_templates, args.templates)
+    gp = brp.GPU(w.templates[0], w.x, 256, 0, 1, False, True, False, True, True, True, True, True,
+                 engine=brp.ENGINE_PLAIN)
+    top_p = brp.Toplist(100)
+    gp.bank_search(w.orbits[:1], U, workloads.SAMPLE_US, brp.Toplist(100))  # warm-up
+    tp0 = time.perf_counter()
+    _, _, plain_gpu_ms = gp.bank_search(w.orbits[:plain_n], U, workloads.SAMPLE_US, top_p)
+    plain_elapsed = time.perf_counter() - tp0
+    plain_tps = plain_n / plain_elapsed
+    gp.close()
+    identical = None
+    if plain_n == args.templates:
+        prow = [(r.bin, r.harmonics, r.period, r.power) for r in top_p.table()]
+        identical = [r[:3] for r in prow] == [r[:3] for r in batched_rows] and \
+            bool(np.allclose([r[3] for r in prow], [r[3] for r in batched_rows], rtol=1e-5, atol=0)) and \
+            [r.bin for r in top_p.final()] == final_bins
+    log(f"[bench] plain: {plain_tps:.1f} templates/s over {plain_n}; toplist identical: {identical}")
+
+    # --------------------------------------------------------------- roofline --
+    B = workloads.algorithmic_bytes(n, U, H)
+    # Per-kernel algorithmic bytes per template: pass 1 reads the series and writes the
+    # N/2-point exchange; pass 2 reads it back and writes the N/2 power bins; the
+    # harmonic kernel reads the power below H.
+    kb = {"pass1_resample_fft": 4 * U + 4 * n, "pass2_fft_power": 4 * n + 2 * n, "harmonic": 4 * H}
+    kms = {"pass1_resample_fft": km[0], "pass2_fft_power": km[1], "harmonic": km[3]}
+    dom = max(kms, key=lambda k: kms[k])
+    per_launch_ms = kms[dom] / groups_per_run
+    achieved = kb[dom] * args.group / (per_launch_ms * 1e-3) if per_launch_ms > 0 else 0.0
+    pipeline_achieved = B * templates_per_s
+
+    # ------------------------------------------------------------ CPU baseline --
+    cpu = None
+    if not args.no_cpu:
+        import oracle_lib
+
+        cores = len(os.sched_getaffinity(0))
+        threads = min(cores, 16)
+        sample = w.templates[:threads]
+        t1 = time.perf_counter()
+        ms = oracle_lib.benchmark_cpu(w.x, sample, threads, 1)
+        spent = time.perf_counter() - t1
+        reps = max(1, int(args.cpu_seconds / max(spent, 1e-3)) - 1)
+        if reps > 1:
+            ms = oracle_lib.benchmark_cpu(w.x, sample, threads, min(reps, 5))
+        cpu = {"value": round(len(sample) / (ms * 1e-3), 3), "unit": "templates/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{len(sample)} bank templates at N={n} (one per thread), resample + FP32 Stockham FFT "
+                         f"(not FFTW) + power + harmonic + select, median of repetitions after one warm-up"}
+        log(f"[bench] cpu baseline {cpu}")
+
+    out = {
+        "metric": "templates/sec at N=2^22, batched vs plain (1 GPU), identical toplist; % roofline",
+        "value": round(templates_per_s, 3),
+        "unit": "templates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"bank of {args.templates} orbital templates x N={n} series (bank semantics)",
+                   "templates": args.templates, "n": n, "batch": args.batch, "group": args.group,
+                   "graph": not args.no_graph},
+        "plain_templates_per_s": round(plain_tps, 3),
+        "batched_over_plain": round(templates_per_s / plain_tps, 3),
+        "toplist_identical": identical,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "per_launch_ms": round(per_launch_ms, 4), "templates_per_launch": args.group,
+                     "algorithmic_bytes_per_template": kb[dom]},
+        "pipeline_roofline": {"bytes_per_template": B, "achieved_GBps": round(pipeline_achieved / 1e9, 2),
+                              "frac": round(pipeline_achieved / HBM_PEAK, 4)},
+        "kernel_ms_last_run": {k: round(v, 4) for k, v in kms.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

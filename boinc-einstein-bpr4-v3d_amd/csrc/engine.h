@@ -1,0 +1,83 @@
+// engine.h -- interfaces between the handle (engine.hip) and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "brp_hip.h"
+
+namespace brp_engine {
+
+// ----------------------------------------------------------------- plain --
+struct PlainBuffers {
+    uint32_t n = 0, fft_n = 0, packed = 0;
+    const float* x = nullptr;        // observation, unpadded samples
+    const float2* lut = nullptr;     // 65 (sin, cos) segments
+    float* raw = nullptr;            // resampled samples, i < steps
+    int32_t* index = nullptr;        // resample source index (snapshot only, else null)
+    double* partial = nullptr;       // per-block FP64 sums
+    float* mean = nullptr;           // pad value
+    float2* a = nullptr;             // FFT ping
+    float2* scratch = nullptr;       // FFT pong
+    float2* X = nullptr;             // n/2 + 1 spectrum bins
+    const float2* tw_fft = nullptr;  // W_fft_n^t, t < fft_n
+    const float2* tw_real = nullptr; // W_n^k, k <= n/2
+};
+
+void plain_clear_slots(hipStream_t st, uint32_t* results, uint32_t slot_words, uint32_t nslots);
+void plain_resample(hipStream_t st, const PlainBuffers& b, const brp_params& p, uint32_t* slot);
+float2* plain_fft(hipStream_t st, const PlainBuffers& b, bool radix8, bool radix4);
+void plain_spectrum(hipStream_t st, const PlainBuffers& b, const float2* z, bool fused_power, float* power);
+void plain_power(hipStream_t st, const PlainBuffers& b, float* power);
+void plain_harmonic(hipStream_t st, const float* power, float* summed, const brp_params& p, bool scatter,
+                    uint32_t* slot);
+
+// ----------------------------------------------------------------- fused --
+// The half-length complex transform of M = N/2 points is split M = R x C.  Pass 1: for
+// each row r, resample+pack z[r + R c] (c < C), FFT over c, multiply by W_M^{r k2}, store
+// Y[r][k2].  Pass 2: for each column k2 (with its mirror column C - k2), FFT over r,
+// real split, pad correction, power P[k1 C + k2].  Harmonic: dyadic sums + candidates.
+struct FusedShape {
+    uint32_t n = 0, m = 0, c = 0, r = 0;  // N, M = N/2, pass-1 length C, pass-2 length R
+    uint32_t s1 = 0, s2 = 0;              // rows per pass-1 block, lower columns per pass-2 block
+    uint32_t blocks1 = 0, blocks2 = 0;    // blocks per template
+    uint32_t tile = 1024;                 // harmonic bins per block
+};
+
+struct FusedArgs {
+    const brp_params* params = nullptr;  // per template slot (device)
+    const float* x = nullptr;
+    const float2* lut = nullptr;
+    const float2* tw_c = nullptr;    // W_C^t, t < C
+    const float2* tw_r = nullptr;    // W_R^t, t < R
+    const float2* twm_lo = nullptr;  // W_M^t, t < 2048
+    const float2* twm_hi = nullptr;  // W_M^{2048 q}
+    const float2* twn_lo = nullptr;  // W_N^t, t < 2048
+    const float2* twn_hi = nullptr;  // W_N^{2048 q}
+    float2* Y = nullptr;             // per group slot: M complex
+    float* P = nullptr;              // per group slot: M floats
+    double* partial = nullptr;       // per group slot: blocks1 doubles
+    float* mean = nullptr;           // per group slot
+    uint32_t* results = nullptr;     // per batch slot
+    uint32_t slot_words = 0;
+    uint32_t first = 0;              // batch slot of group slot 0
+    uint32_t count = 0;              // templates in this group
+    uint32_t fp64_phase = 0;
+    // snapshot (per batch slot, or null)
+    float* snap_resampled = nullptr;  // N floats
+    int32_t* snap_index = nullptr;    // N int32
+    float2* snap_X = nullptr;         // M + 1 complex
+    float* snap_summed = nullptr;     // 5 F floats
+    uint32_t snap_f = 0;
+};
+
+bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* shape);
+// One launch each covers all `count` templates of the group.
+void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a);
+void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a);
+void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles);
+// Copies {count, flags} and the first `keep` candidates of each of `nslots` slots into a
+// dense staging area of (keep + 1) x 16 bytes per slot.
+void pack_results(hipStream_t st, const uint32_t* results, uint32_t slot_words, uint32_t nslots, uint32_t keep,
+                  uint32_t* staging);
+
+}  // namespace brp_engine

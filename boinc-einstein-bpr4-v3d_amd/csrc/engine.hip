@@ -1,0 +1,606 @@
+// engine.hip -- the brp_hip handle: device memory, tables, launch sequences, result
+// readback and the bank loop behind the C ABI of include/brp_hip.h.  It takes the place
+// of the reference's GPU::Impl (src/vulkan.cpp:21-277) and GPU methods
+// (src/vulkan.cpp:278-406), and of the bank loop of src/workunit_main.cpp:77-136.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "brp_hip.h"
+#include "engine.h"
+#include "errors.h"
+#include "lut.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Failure : std::runtime_error {
+    int code;
+    Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Failure(BRP_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(expr) hip_check((expr), #expr)
+
+void require(bool ok, const char* message) {
+    if (!ok) throw Failure(BRP_E_INVALID, message);
+}
+
+double process_cpu_ms() {
+    timespec t{};
+    clock_gettime(CLOCK_PROCESS_CPUTIME_ID, &t);
+    return t.tv_sec * 1000.0 + t.tv_nsec / 1e6;
+}
+
+bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// W_len^t = e^{-2 pi i t / len} for t in [first, first + count) step `stride`,
+// evaluated in double and rounded once to FP32.
+std::vector<float2> twiddles(uint64_t len, uint64_t count, uint64_t stride = 1) {
+    std::vector<float2> w(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        const uint64_t t = (i * stride) % len;
+        const double a = -kTwoPi * double(t) / double(len);
+        w[i] = make_float2(float(std::cos(a)), float(std::sin(a)));
+    }
+    return w;
+}
+
+template <class F>
+int guarded(F&& body) {
+    try {
+        body();
+        return BRP_OK;
+    } catch (const Failure& f) {
+        g_error = f.what();
+        return f.code;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return BRP_E_RUNTIME;
+    }
+}
+
+}  // namespace
+
+namespace brp {
+int fail(int code, const std::string& message) {
+    g_error = message;
+    return code;
+}
+}  // namespace brp
+
+struct brp_hip {
+    brp_params base{};
+    brp_options opt{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocations;
+    uint64_t allocated = 0;
+    std::vector<brp_params> templates;
+
+    float* x = nullptr;
+    float2* lut = nullptr;
+    uint32_t* results = nullptr;
+    uint32_t slot_words = 0;
+    uint32_t keep = 64;
+    uint32_t* staging_dev = nullptr;
+    uint32_t* staging_host = nullptr;
+    brp_params* params_dev = nullptr;
+    brp_params* params_host = nullptr;
+
+    // plain engine
+    brp_engine::PlainBuffers pb;
+    float* power_buf = nullptr;  // n/2 power + 5 F summed (the reference's buffer 4 layout)
+
+    // fused engine
+    brp_engine::FusedShape shape;
+    brp_engine::FusedArgs fa;
+    uint32_t group = 1;
+    uint32_t tiles = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_count = 0;
+    bool graph_events = true;
+
+    std::vector<hipEvent_t> events;
+    std::vector<double> stage_ms_per_template;  // 4 per template of the last run
+    double kernel_ms[4] = {0, 0, 0, 0};
+    bool ran = false;
+    std::vector<std::vector<brp_candidate>> extra;  // full lists fetched beyond `keep`
+
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        HIPC(hipMalloc(&p, bytes));
+        allocations.push_back(p);
+        allocated += bytes;
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        HIPC(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    hipEvent_t event(size_t i) {
+        while (events.size() <= i) {
+            hipEvent_t e;
+            HIPC(hipEventCreate(&e));
+            events.push_back(e);
+        }
+        return events[i];
+    }
+    ~brp_hip() {
+        if (stream) hipStreamSynchronize(stream);
+        if (graph_exec) hipGraphExecDestroy(graph_exec);
+        for (hipEvent_t e : events) hipEventDestroy(e);
+        for (void* p : allocations) hipFree(p);
+        if (staging_host) hipHostFree(staging_host);
+        if (params_host) hipHostFree(params_host);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void check_options(const brp_params& p, const brp_options& o) {
+    require(o.engine == BRP_ENGINE_PLAIN || o.engine == BRP_ENGINE_FUSED, "unknown engine");
+    // src/vulkan.cpp:279-283
+    require(!o.fused_power || o.packed, "fused power requires packed real FFT");
+    require(!o.radix8_store || o.radix8, "paired-store radix 8 requires radix 8");
+    require(!o.radix8 || (o.stockham && !o.tile), "radix8 requires Stockham without a shared tile");
+    // src/vulkan.cpp:116-122
+    const uint32_t fft_n = o.packed ? p.n / 2 : p.n;
+    const uint32_t fft_base = pow2(fft_n) ? fft_n : fft_n / 3;
+    require(o.tile <= fft_base, "FFT tile exceeds radix-2 block");
+    require(!(o.stockham && o.tile), "Stockham requires --tile 0");
+    require(o.workgroup && o.workgroup <= 256 && o.batch && o.batch <= 4096 &&
+                (!o.tile || (pow2(o.tile) && o.tile >= 64 && o.tile <= 4096 && o.tile <= p.n)),
+            "invalid workgroup, FFT tile, or batch");
+    require(o.precision == 0, "split-FP16 precision variant is not available in this build");
+}
+
+void check_template(const brp_params& base, const brp_params& next) {
+    // src/vulkan.cpp:270-276
+    require((next.flags & 6) == (base.flags & 6), "arithmetic specialization changed; rebuild pipelines");
+    require(next.n == base.n && next.unpadded == base.unpadded && next.fundamental == base.fundamental &&
+                next.harmonic == base.harmonic && next.capacity == base.capacity && next.steps &&
+                next.steps <= next.unpadded,
+            "template dimensions changed; rebuild resources");
+    require(next.window <= next.fundamental && next.harmonic <= next.n / 2 && next.fundamental <= next.n / 2,
+            "template search range outside the spectrum");
+}
+
+void create_plain(brp_hip& h) {
+    const brp_params& p = h.base;
+    auto& b = h.pb;
+    b.n = p.n;
+    b.packed = h.opt.packed ? 1u : 0u;
+    b.fft_n = b.packed ? p.n / 2 : p.n;
+    b.x = h.x;
+    b.lut = h.lut;
+    b.raw = h.alloc<float>(p.unpadded);
+    b.index = h.opt.snapshot ? h.alloc<int32_t>(p.unpadded) : nullptr;
+    b.partial = h.alloc<double>((p.unpadded + 255) / 256);
+    b.mean = h.alloc<float>(1);
+    b.a = h.alloc<float2>(b.fft_n + 1);
+    b.scratch = h.alloc<float2>(b.fft_n + 1);
+    b.X = h.alloc<float2>(p.n / 2 + 1);
+    b.tw_fft = h.upload(twiddles(b.fft_n, b.fft_n));
+    b.tw_real = h.upload(twiddles(p.n, p.n / 2 + 1));
+    h.power_buf = h.alloc<float>(size_t(p.n / 2) + 5ull * p.fundamental);
+    HIPC(hipMemset(h.power_buf, 0, sizeof(float) * (size_t(p.n / 2) + 5ull * p.fundamental)));
+}
+
+void create_fused(brp_hip& h) {
+    const brp_params& p = h.base;
+    require(brp_engine::fused_supported(p.n, h.opt.split_c, &h.shape),
+            "fused engine: unsupported series length (N = 2^13..2^23 or 3*2^15..3*2^22)");
+    const auto& s = h.shape;
+    h.group = h.opt.snapshot ? h.opt.batch : std::min<uint32_t>(h.opt.batch, h.opt.group ? h.opt.group : 8);
+    auto& a = h.fa;
+    a.x = h.x;
+    a.lut = h.lut;
+    a.tw_c = h.upload(twiddles(s.c, s.c));
+    a.tw_r = h.upload(twiddles(s.r, s.r));
+    a.twm_lo = h.upload(twiddles(s.m, 2048));
+    a.twm_hi = h.upload(twiddles(s.m, (s.m + 2047) / 2048, 2048));
+    a.twn_lo = h.upload(twiddles(s.n, 2048));
+    a.twn_hi = h.upload(twiddles(s.n, (s.n + 2047) / 2048, 2048));
+    a.Y = h.alloc<float2>(size_t(h.group) * s.m);
+    a.P = h.alloc<float>(size_t(h.group) * s.m);
+    a.partial = h.alloc<double>(size_t(h.group) * s.blocks1);
+    a.mean = h.alloc<float>(h.opt.batch);
+    a.results = h.results;
+    a.slot_words = h.slot_words;
+    a.params = h.params_dev;
+    a.fp64_phase = h.opt.fp64_phase;
+    if (h.opt.snapshot) {
+        a.snap_resampled = h.alloc<float>(size_t(h.opt.batch) * s.n);
+        a.snap_index = h.alloc<int32_t>(size_t(h.opt.batch) * s.n);
+        a.snap_X = h.alloc<float2>(size_t(h.opt.batch) * (s.m + 1));
+        a.snap_summed = h.alloc<float>(size_t(h.opt.batch) * 5 * p.fundamental);
+        a.snap_f = p.fundamental;
+    }
+    // Harmonic tiles cover every fundamental's rounding interval: centres j 2^h < 16 F,
+    // intervals that start below H.
+    const uint64_t reach = std::min<uint64_t>(16ull * p.fundamental, uint64_t(p.harmonic) + 8);
+    h.tiles = uint32_t((reach + 1023) / 1024);
+}
+
+// The plain launch sequence for slot r (src/vulkan.cpp:221-266), with events at the
+// four stage boundaries.
+void enqueue_plain(brp_hip& h, uint32_t r) {
+    const brp_params& p = h.templates[r];
+    hipStream_t st = h.stream;
+    uint32_t* slot = h.results + size_t(r) * h.slot_words;
+    brp_engine::plain_clear_slots(st, slot, h.slot_words, 1);
+    HIPC(hipEventRecord(h.event(5 * r), st));
+    brp_engine::plain_resample(st, h.pb, p, slot);
+    HIPC(hipEventRecord(h.event(5 * r + 1), st));
+    const bool fused_power = h.opt.fused_power != 0;
+    float2* z = brp_engine::plain_fft(st, h.pb, h.opt.radix8 != 0, h.opt.radix4 != 0 || h.opt.radix8 != 0);
+    brp_engine::plain_spectrum(st, h.pb, z, fused_power, h.power_buf);
+    HIPC(hipEventRecord(h.event(5 * r + 2), st));
+    if (!fused_power || !h.pb.packed) brp_engine::plain_power(st, h.pb, h.power_buf);
+    HIPC(hipEventRecord(h.event(5 * r + 3), st));
+    brp_engine::plain_harmonic(st, h.power_buf, h.power_buf + p.n / 2, p, h.opt.scatter != 0, slot);
+    HIPC(hipEventRecord(h.event(5 * r + 4), st));
+}
+
+// The fused sequence: every group of `group` templates in three launches.
+void enqueue_fused(brp_hip& h, uint32_t count, bool capturing) {
+    hipStream_t st = h.stream;
+    brp_engine::plain_clear_slots(st, h.results, h.slot_words, count);
+    const bool ev = !capturing || h.graph_events;
+    auto record = [&](size_t i) {
+        if (!ev) return;
+        if (capturing) HIPC(hipEventRecordWithFlags(h.event(i), st, hipEventRecordExternal));
+        else HIPC(hipEventRecord(h.event(i), st));
+    };
+    uint32_t gi = 0;
+    for (uint32_t first = 0; first < count; first += h.group, ++gi) {
+        brp_engine::FusedArgs a = h.fa;
+        a.first = first;
+        a.count = std::min(h.group, count - first);
+        record(4 * gi);
+        brp_engine::fused_pass1(st, h.shape, a);
+        record(4 * gi + 1);
+        brp_engine::fused_pass2(st, h.shape, a);
+        record(4 * gi + 2);
+        brp_engine::fused_harmonic(st, h.shape, a, h.tiles);
+        record(4 * gi + 3);
+    }
+    brp_engine::pack_results(st, h.results, h.slot_words, count, h.keep, h.staging_dev);
+}
+
+void run_fused(brp_hip& h, uint32_t count) {
+    if (!h.opt.graph || h.opt.snapshot) {
+        enqueue_fused(h, count, false);
+        return;
+    }
+    if (!h.graph_exec || h.graph_count != count) {
+        if (h.graph_exec) { HIPC(hipGraphExecDestroy(h.graph_exec)); h.graph_exec = nullptr; }
+        hipGraph_t graph = nullptr;
+        auto capture = [&](bool events) {
+            h.graph_events = events;
+            HIPC(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_fused(h, count, true);
+            } catch (...) {
+                hipGraph_t dropped = nullptr;
+                hipStreamEndCapture(h.stream, &dropped);
+                if (dropped) hipGraphDestroy(dropped);
+                throw;
+            }
+            HIPC(hipStreamEndCapture(h.stream, &graph));
+        };
+        try {
+            capture(true);
+        } catch (const Failure&) {
+            (void)hipGetLastError();
+            capture(false);  // this runtime cannot record event nodes: time the whole replay
+        }
+        HIPC(hipGraphInstantiate(&h.graph_exec, graph, nullptr, nullptr, 0));
+        HIPC(hipGraphDestroy(graph));
+        h.graph_count = count;
+    }
+    HIPC(hipGraphLaunch(h.graph_exec, h.stream));
+}
+
+void fill_timings(brp_hip& h, uint32_t count) {
+    h.stage_ms_per_template.assign(4 * size_t(count), 0.0);
+    std::fill(h.kernel_ms, h.kernel_ms + 4, 0.0);
+    float ms = 0;
+    if (h.opt.engine == BRP_ENGINE_PLAIN) {
+        for (uint32_t r = 0; r < count; ++r)
+            for (uint32_t s = 0; s < 4; ++s) {
+                HIPC(hipEventElapsedTime(&ms, h.event(5 * r + s), h.event(5 * r + s + 1)));
+                h.stage_ms_per_template[4 * r + s] = ms;
+                h.kernel_ms[s] += ms;
+            }
+        return;
+    }
+    if (h.opt.graph && !h.opt.snapshot && !h.graph_events) return;
+    const uint32_t groups = (count + h.group - 1) / h.group;
+    for (uint32_t gi = 0; gi < groups; ++gi) {
+        const uint32_t first = gi * h.group, n = std::min(h.group, count - first);
+        double stage[4] = {0, 0, 0, 0};
+        HIPC(hipEventElapsedTime(&ms, h.event(4 * gi), h.event(4 * gi + 1)));
+        stage[0] = ms;
+        HIPC(hipEventElapsedTime(&ms, h.event(4 * gi + 1), h.event(4 * gi + 2)));
+        stage[1] = ms;
+        HIPC(hipEventElapsedTime(&ms, h.event(4 * gi + 2), h.event(4 * gi + 3)));
+        stage[3] = ms;
+        for (int s = 0; s < 4; ++s) h.kernel_ms[s] += stage[s];
+        for (uint32_t r = first; r < first + n; ++r)
+            for (int s = 0; s < 4; ++s) h.stage_ms_per_template[4 * r + s] = stage[s] / n;
+    }
+}
+
+void read_slot(brp_hip& h, uint32_t slot, std::vector<brp_candidate>& out) {
+    require(h.ran, "run() has not been called");
+    require(slot < h.templates.size(), "candidate slot outside executed batch");
+    const uint32_t* rec = h.staging_host + size_t(slot) * (h.keep + 1) * 4;
+    const uint32_t count = rec[0], flags = rec[1];
+    const brp_params& p = h.templates[slot];
+    if (flags || count > p.capacity)
+        throw Failure(BRP_E_OVERFLOW, "GPU invalid resample index or candidate overflow; no valid result");
+    out.resize(count);
+    if (count <= h.keep) {
+        std::memcpy(out.data(), rec + 4, sizeof(brp_candidate) * count);
+    } else {
+        HIPC(hipMemcpy(out.data(), h.results + size_t(slot) * h.slot_words + 4, sizeof(brp_candidate) * count,
+                       hipMemcpyDeviceToHost));
+    }
+    std::sort(out.begin(), out.end(), [](const brp_candidate& a, const brp_candidate& b) {
+        return a.level != b.level ? a.level < b.level : a.bin < b.bin;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* brp_last_error(void) { return g_error.c_str(); }
+
+int brp_hip_create(const brp_params* p, const float* input, size_t count, const brp_options* opt, brp_hip** out) {
+    return guarded([&] {
+        require(p && input && opt && out, "null argument");
+        *out = nullptr;
+        check_options(*p, *opt);
+        require(count == p->unpadded, "input length mismatch");
+        require(p->n >= 64 && p->steps && p->steps <= p->unpadded && p->unpadded <= p->n, "invalid template");
+        int ndev = 0;
+        HIPC(hipGetDeviceCount(&ndev));
+        require(ndev > 0, "no HIP device");
+        auto h = std::make_unique<brp_hip>();
+        h->base = *p;
+        h->opt = *opt;
+        if (!h->opt.workgroup) h->opt.workgroup = 256;
+        if (opt->engine == BRP_ENGINE_FUSED) h->base.flags |= 1u | 64u;  // packed, fused power
+        HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->x = h->alloc<float>(p->unpadded);
+        HIPC(hipMemcpy(h->x, input, sizeof(float) * p->unpadded, hipMemcpyHostToDevice));
+        std::vector<float2> lut(65);
+        for (int i = 0; i < 65; ++i) lut[i] = make_float2(brp_lut::kTable[2 * i], brp_lut::kTable[2 * i + 1]);
+        h->lut = h->upload(lut);
+        h->slot_words = 4 + 4 * p->capacity;
+        h->results = h->alloc<uint32_t>(size_t(opt->batch) * h->slot_words);
+        HIPC(hipMemset(h->results, 0, sizeof(uint32_t) * size_t(opt->batch) * h->slot_words));
+        h->staging_dev = h->alloc<uint32_t>(size_t(opt->batch) * (h->keep + 1) * 4);
+        HIPC(hipHostMalloc(reinterpret_cast<void**>(&h->staging_host), sizeof(uint32_t) * size_t(opt->batch) * (h->keep + 1) * 4, hipHostMallocDefault));
+        h->params_dev = h->alloc<brp_params>(opt->batch);
+        HIPC(hipHostMalloc(reinterpret_cast<void**>(&h->params_host), sizeof(brp_params) * opt->batch, hipHostMallocDefault));
+        if (opt->engine == BRP_ENGINE_PLAIN) create_plain(*h);
+        else create_fused(*h);
+        h->templates.assign(1, *p);
+        *out = h.release();
+    });
+}
+
+int brp_hip_destroy(brp_hip* h) {
+    return guarded([&] { delete h; });
+}
+
+int brp_hip_set_templates(brp_hip* h, const brp_params* p, size_t count) {
+    return guarded([&] {
+        require(h && p, "null argument");
+        require(count && count <= h->opt.batch, "template batch exceeds retained result capacity");
+        for (size_t i = 0; i < count; ++i) check_template(h->base, p[i]);
+        h->templates.assign(p, p + count);
+        h->ran = false;
+    });
+}
+
+int brp_hip_run(brp_hip* h, brp_timings* t, double* template_stage_ms) {
+    return guarded([&] {
+        require(h, "null handle");
+        const uint32_t count = uint32_t(h->templates.size());
+        const auto start = std::chrono::steady_clock::now();
+        const double cpu0 = process_cpu_ms();
+        for (uint32_t i = 0; i < count; ++i) {
+            h->params_host[i] = h->templates[i];
+            h->params_host[i].result_slot = i;
+        }
+        const auto submit0 = std::chrono::steady_clock::now();
+        HIPC(hipMemcpyAsync(h->params_dev, h->params_host, sizeof(brp_params) * count, hipMemcpyHostToDevice, h->stream));
+        if (h->opt.engine == BRP_ENGINE_PLAIN) {
+            if (h->opt.snapshot) {
+                HIPC(hipMemsetAsync(h->power_buf + h->base.n / 2, 0, sizeof(float) * 5ull * h->base.fundamental, h->stream));
+            }
+            for (uint32_t r = 0; r < count; ++r) enqueue_plain(*h, r);
+            brp_engine::pack_results(h->stream, h->results, h->slot_words, count, h->keep, h->staging_dev);
+        } else {
+            if (h->opt.snapshot) {
+                HIPC(hipMemsetAsync(h->fa.snap_summed, 0, sizeof(float) * size_t(count) * 5 * h->fa.snap_f, h->stream));
+                HIPC(hipMemsetAsync(h->fa.snap_X, 0, sizeof(float2) * size_t(count) * (h->shape.m + 1), h->stream));
+            }
+            run_fused(*h, count);
+        }
+        const double submit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - submit0).count();
+        HIPC(hipMemcpyAsync(h->staging_host, h->staging_dev, sizeof(uint32_t) * size_t(count) * (h->keep + 1) * 4,
+                            hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        h->ran = true;
+        fill_timings(*h, count);
+        if (t) {
+            t->submit_ms = submit_ms;
+            for (int s = 0; s < 4; ++s) t->stage_ms[s] = h->kernel_ms[s];
+            t->cpu_ms = process_cpu_ms() - cpu0;
+            t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+        }
+        if (template_stage_ms)
+            std::copy(h->stage_ms_per_template.begin(), h->stage_ms_per_template.end(), template_stage_ms);
+    });
+}
+
+int brp_hip_candidates(brp_hip* h, uint32_t slot, brp_candidate* out, size_t cap, size_t* count) {
+    return guarded([&] {
+        require(h && count, "null argument");
+        std::vector<brp_candidate> v;
+        read_slot(*h, slot, v);
+        *count = v.size();
+        require(v.size() <= cap || !out, "candidate buffer too small");
+        if (out) std::copy(v.begin(), v.end(), out);
+    });
+}
+
+int brp_hip_snapshot(brp_hip* h, uint32_t slot, float* resampled, int32_t* index, float* fft, float* power,
+                     float* summed) {
+    return guarded([&] {
+        require(h && h->ran, "run() has not been called");
+        require(slot < h->templates.size(), "snapshot slot outside executed batch");
+        const brp_params& p = h->templates[slot];
+        const uint32_t n = p.n, half = n / 2;
+        if (h->opt.engine == BRP_ENGINE_PLAIN) {
+            // The plain engine runs a batch sequentially on shared buffers (like the reference),
+            // so its arrays hold the last template of the batch.
+            require(slot + 1 == h->templates.size(), "plain engine snapshots hold the last template of the batch");
+            if (resampled) {
+                HIPC(hipMemcpy(resampled, h->pb.raw, sizeof(float) * p.steps, hipMemcpyDeviceToHost));
+                float mean = 0;
+                HIPC(hipMemcpy(&mean, h->pb.mean, sizeof(float), hipMemcpyDeviceToHost));
+                std::fill(resampled + p.steps, resampled + n, mean);
+            }
+            if (index) {
+                require(h->pb.index, "index snapshot needs options.snapshot");
+                HIPC(hipMemcpy(index, h->pb.index, sizeof(int32_t) * p.steps, hipMemcpyDeviceToHost));
+            }
+            if (fft) HIPC(hipMemcpy(fft, h->pb.X, sizeof(float2) * (half + 1), hipMemcpyDeviceToHost));
+            if (power) HIPC(hipMemcpy(power, h->power_buf, sizeof(float) * half, hipMemcpyDeviceToHost));
+            if (summed) HIPC(hipMemcpy(summed, h->power_buf + half, sizeof(float) * 5ull * p.fundamental, hipMemcpyDeviceToHost));
+            return;
+        }
+        require(h->opt.snapshot, "fused engine snapshots need options.snapshot");
+        const auto& a = h->fa;
+        if (resampled) {
+            HIPC(hipMemcpy(resampled, a.snap_resampled + size_t(slot) * n, sizeof(float) * p.steps, hipMemcpyDeviceToHost));
+            float mean = 0;
+            HIPC(hipMemcpy(&mean, a.mean + slot, sizeof(float), hipMemcpyDeviceToHost));
+            std::fill(resampled + p.steps, resampled + n, mean);
+        }
+        if (index) HIPC(hipMemcpy(index, a.snap_index + size_t(slot) * n, sizeof(int32_t) * p.steps, hipMemcpyDeviceToHost));
+        if (fft) HIPC(hipMemcpy(fft, a.snap_X + size_t(slot) * (half + 1), sizeof(float2) * (half + 1), hipMemcpyDeviceToHost));
+        if (power) HIPC(hipMemcpy(power, a.P + size_t(slot) * h->shape.m, sizeof(float) * half, hipMemcpyDeviceToHost));
+        if (summed)
+            HIPC(hipMemcpy(summed, a.snap_summed + size_t(slot) * 5 * a.snap_f, sizeof(float) * 5ull * p.fundamental,
+                           hipMemcpyDeviceToHost));
+    });
+}
+
+size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap) {
+    (void)h;
+    std::ostringstream s;
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        int rt = 0;
+        hipRuntimeGetVersion(&rt);
+        s << prop.name << "; " << prop.gcnArchName << "; CUs=" << prop.multiProcessorCount
+          << "; wavefront=" << prop.warpSize << "; LDS/block=" << prop.sharedMemPerBlock
+          << "; HBM=" << (prop.totalGlobalMem >> 20) << " MiB; clock_khz=" << prop.clockRate << "; HIP runtime " << rt;
+    } else {
+        s << "no HIP device";
+    }
+    const std::string text = s.str();
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, text.size());
+        std::memcpy(buf, text.data(), n);
+        buf[n] = 0;
+    }
+    return text.size();
+}
+
+uint64_t brp_hip_allocated_bytes(const brp_hip* h) { return h ? h->allocated : 0; }
+
+size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap) {
+    (void)h;
+    const std::string text =
+        "{\"available\":false,\"reason\":\"hardware counters are collected out of process with rocprofv3 --pmc\"}";
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, text.size());
+        std::memcpy(buf, text.data(), n);
+        buf[n] = 0;
+    }
+    return text.size();
+}
+
+int brp_hip_kernel_ms(brp_hip* h, double out[4]) {
+    return guarded([&] {
+        require(h && out && h->ran, "run() has not been called");
+        std::copy(h->kernel_ms, h->kernel_ms + 4, out);
+    });
+}
+
+int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint32_t samples, double sample_us,
+                        double f0, double false_alarm, uint32_t window, brp_toplist* top, brp_params* templates_out,
+                        uint32_t* cand_count, double* gpu_ms) {
+    return guarded([&] {
+        require(h && orbits && top, "null argument");
+        // Parameters that do not depend on the toplist, once per template.
+        std::vector<brp_params> base(count);
+        for (size_t i = 0; i < count; ++i)
+            if (int rc = brp_search_parameters(h->base.n, samples, sample_us, &orbits[i], f0, false_alarm, window, &base[i]))
+                throw Failure(rc, brp_last_error());
+        const size_t batch = h->opt.batch;
+        double total_gpu = 0;
+        std::vector<brp_params> group;
+        std::vector<brp_candidate> cand;
+        for (size_t index = 0; index < count; index += batch) {
+            const size_t n = std::min(batch, count - index);
+            group.assign(base.begin() + index, base.begin() + index + n);
+            for (auto& p : group) brp_toplist_thresholds(top, &p);  // frozen at the batch start
+            if (int rc = brp_hip_set_templates(h, group.data(), n)) throw Failure(rc, brp_last_error());
+            brp_timings t{};
+            if (int rc = brp_hip_run(h, &t, nullptr)) throw Failure(rc, brp_last_error());
+            for (int s = 0; s < 4; ++s) total_gpu += t.stage_ms[s];
+            for (size_t k = 0; k < n; ++k) {
+                brp_params now = base[index + k];
+                brp_toplist_thresholds(top, &now);  // the toplist as of this template
+                read_slot(*h, uint32_t(k), cand);
+                if (int rc = brp_toplist_retain(top, cand.data(), cand.size(), &orbits[index + k], &now))
+                    throw Failure(rc, brp_last_error());
+                if (templates_out) templates_out[index + k] = now;
+                if (cand_count) cand_count[index + k] = uint32_t(cand.size());
+            }
+        }
+        if (gpu_ms) *gpu_ms = total_gpu;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,472 @@
+// fused.hip -- the fused engine for gfx950.
+//
+// Per template the real series of N samples is packed into M = N/2 complex points and
+// transformed as a two-pass four-step FFT with M = R x C:
+//
+//   pass 1  (block = S1 consecutive rows r, all c):  resample + pack z[r + R c] straight
+//           from the observation (the bit-exact index rule of src/science.cpp:75-85),
+//           FFT over c in registers/LDS, multiply by W_M^{r k2}, store Y[r][k2] (rows are
+//           contiguous, so every store is a full line).  The pad is left at zero here and
+//           each block writes an FP64 partial sum of its samples.
+//   pass 2  (block = S2 columns k2 plus their mirror columns C - k2, all r):  FFT over r,
+//           giving Z[k1 C + k2] and Z[M - k] in the same block, then the real split
+//           X[k] = 1/2 [(Z_k + conj Z_{M-k}) - i W_N^k (Z_k - conj Z_{M-k})], the mean pad
+//           added by linearity, X += mu U with U[k] = sum_{t >= steps} W_N^{tk}
+//           = (W_N^{k steps} - 1) / (1 - W_N^k), and the power |X|^2 / N.
+//   harmonic (block = 1024 bins + 8 halo):  dyadic sums of every bin into LDS, then the
+//           maximum over each fundamental's rounding interval, threshold test and
+//           candidate append (the gather result of shaders/harmonic.comp:14-36 without
+//           recomputing sums per level).
+//
+// One launch of each kernel covers every template of a group, blocks mapped so that
+// consecutive work items stay on one XCD (8 XCDs, block b runs on XCD b mod 8).
+#include "device.h"
+#include "engine.h"
+
+#pragma clang fp contract(off)
+
+using namespace brp_dev;
+using brp_engine::FusedArgs;
+using brp_engine::FusedShape;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTile = 1024;  // harmonic bins per block
+
+struct Radices { int count; int r[5]; };
+
+constexpr Radices radices_for(int L) {
+    switch (L) {
+        case 64: return {2, {16, 4}};
+        case 128: return {2, {16, 8}};
+        case 256: return {2, {16, 16}};
+        case 512: return {3, {8, 8, 8}};
+        case 1024: return {3, {16, 16, 4}};
+        case 2048: return {3, {16, 16, 8}};
+        case 4096: return {3, {16, 16, 16}};
+        case 384: return {3, {3, 16, 8}};
+        case 768: return {3, {3, 16, 16}};
+        case 1536: return {4, {3, 8, 8, 8}};
+        case 3072: return {4, {3, 16, 16, 4}};
+        default: return {0, {0}};
+    }
+}
+
+template <int L, int ST>
+constexpr int ns_before() {
+    int p = 1;
+    for (int i = 0; i < ST; ++i) p *= radices_for(L).r[i];
+    return p;
+}
+
+template <int L>
+constexpr bool plan_ok(int e) {
+    constexpr Radices pl = radices_for(L);
+    int prod = 1;
+    for (int i = 0; i < pl.count; ++i) {
+        if (e % pl.r[i]) return false;
+        prod *= pl.r[i];
+    }
+    return pl.count > 0 && prod == L;
+}
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t grid) {
+    if (grid % 8u) return bid;
+    return (bid % 8u) * (grid / 8u) + bid / 8u;
+}
+
+// LDS slot of element l of sequence s (sequences interleaved, element-major).
+template <int S>
+__device__ __forceinline__ int lds_at(int l, int s) { return l * S + s; }
+
+// Stockham stage ST of an L-point FFT over S sequences held by T threads, E = L S / T
+// elements each.  On entry v[u R + q] holds input q of butterfly u = (s, j), b = t + T u,
+// s = b mod S, j = b / S, i.e. element j + q L/R.  On exit of the last stage v[u R + q]
+// holds output element j + q L/R of sequence s.
+template <int L, int S, int T, int ST>
+__device__ __forceinline__ void fft_stages(cf* v, float2* lds, const float2* __restrict__ tw, int t) {
+    constexpr Radices pl = radices_for(L);
+    constexpr int R = pl.r[ST];
+    constexpr int NS = ns_before<L, ST>();
+    constexpr int E = L * S / T;
+    constexpr int U = E / R;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int b = t + T * u;
+        const int j = b / S;
+        const int k = j % NS;
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) v[u * R + q] = mul(v[u * R + q], load(tw + k * q * (L / (NS * R))));
+        }
+        dft<R>(v + u * R);
+    }
+    if constexpr (ST + 1 < pl.count) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = t + T * u;
+            const int s = b % S, j = b / S, k = j % NS;
+            const int out = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) store(lds + lds_at<S>(out + q * NS, s), v[u * R + q]);
+        }
+        __syncthreads();
+        constexpr int R2 = pl.r[ST + 1];
+        constexpr int U2 = E / R2;
+#pragma unroll
+        for (int u = 0; u < U2; ++u) {
+            const int b = t + T * u;
+            const int s = b % S, j = b / S;
+#pragma unroll
+            for (int q = 0; q < R2; ++q) v[u * R2 + q] = load(lds + lds_at<S>(j + q * (L / R2), s));
+        }
+        fft_stages<L, S, T, ST + 1>(v, lds, tw, t);
+    }
+}
+
+__device__ __forceinline__ cf twiddle2(const float2* __restrict__ lo, const float2* __restrict__ hi, uint32_t e) {
+    return mul(load(hi + (e >> 11)), load(lo + (e & 2047u)));
+}
+
+// ------------------------------------------------------------------ pass 1 --
+template <int C, int S, int T>
+__global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
+    constexpr int E = C * S / T;
+    static_assert(plan_ok<C>(E), "pass-1 radix plan does not fit the block");
+    constexpr Radices pl = radices_for(C);
+    constexpr int R0 = pl.r[0], RL = pl.r[pl.count - 1];
+    __shared__ float2 lds[C * S];
+    __shared__ double red[T / 64];
+    const int t = threadIdx.x;
+    const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t g = item / sh.blocks1, rb = item % sh.blocks1;
+    const uint32_t slot = a.first + g;
+    const brp_params& p = a.params[slot];
+    const uint32_t R = sh.r, M = sh.m;
+    const uint32_t r0 = rb * S;
+    const uint32_t steps = p.steps, U = p.unpadded;
+    const Orbit32 o{p.tau, p.omega, p.psi, p.dt, p.step_inv, p.s0, p.flags};
+    const double tau_dt = double(p.tau) / double(p.dt), s0_dt = double(p.tau) * sin(double(p.psi)) / double(p.dt);
+    float* snap_y = a.snap_resampled ? a.snap_resampled + size_t(slot) * sh.n : nullptr;
+    int32_t* snap_j = a.snap_index ? a.snap_index + size_t(slot) * sh.n : nullptr;
+
+    double acc = 0.0;
+    bool bad = false;
+    auto sample = [&](uint32_t i) -> float {
+        if (i >= steps) return 0.0f;
+        const int j = a.fp64_phase ? source_index_fp64(i, tau_dt, double(p.omega), double(p.psi), double(p.dt), s0_dt)
+                                   : source_index(i, o, a.lut);
+        float y = 0.0f;
+        if (j < 0 || uint32_t(j) >= U) bad = true;
+        else y = a.x[j];
+        acc += double(y);
+        if (snap_y) { snap_y[i] = y; snap_j[i] = j; }
+        return y;
+    };
+
+    cf v[E];
+#pragma unroll
+    for (int u = 0; u < E / R0; ++u) {
+        const int b = t + T * u;
+        const int s = b % S, j = b / S;
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+            const uint32_t c = uint32_t(j + q * (C / R0));
+            const uint32_t m = r0 + s + R * c;
+            const float y0 = sample(2 * m);
+            const float y1 = sample(2 * m + 1);
+            v[u * R0 + q] = cf{y0, y1};
+        }
+    }
+    fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
+
+    float2* Y = a.Y + size_t(g) * M;
+#pragma unroll
+    for (int u = 0; u < E / RL; ++u) {
+        const int b = t + T * u;
+        const int s = b % S, j = b / S;
+        const uint32_t r = r0 + s;
+#pragma unroll
+        for (int q = 0; q < RL; ++q) {
+            const uint32_t k2 = uint32_t(j + q * (C / RL));
+            store(Y + size_t(r) * C + k2, mul(v[u * RL + q], twiddle2(a.twm_lo, a.twm_hi, r * k2)));
+        }
+    }
+
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((t & 63) == 0) red[t >> 6] = acc;
+    if (bad) atomicOr(a.results + size_t(slot) * a.slot_words + 1, 2u);
+    __syncthreads();
+    if (t == 0) {
+        double total = 0.0;
+        for (int w = 0; w < T / 64; ++w) total += red[w];
+        a.partial[size_t(g) * sh.blocks1 + rb] = total;
+    }
+}
+
+// cot(theta/2) for W = e^{-i theta} = (wr, wi), 0 < theta < 2 pi, in the better
+// conditioned of its two half-angle forms.
+__device__ __forceinline__ float half_cot(float wr, float wi) {
+    const bool front = wr >= 0.0f;
+    const float num = front ? 1.0f + wr : -wi;
+    const float den = front ? -wi : 1.0f - wr;
+    return __fdividef(num, den);
+}
+
+// ------------------------------------------------------------------ pass 2 --
+template <int R, int S, int T>
+__global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
+    constexpr int SS = 2 * S;
+    constexpr int E = R * SS / T;
+    static_assert(plan_ok<R>(E), "pass-2 radix plan does not fit the block");
+    constexpr Radices pl = radices_for(R);
+    constexpr int R0 = pl.r[0], RL = pl.r[pl.count - 1];
+    __shared__ float2 lds[R * SS];
+    __shared__ double red[T];
+    const int t = threadIdx.x;
+    const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t g = item / sh.blocks2, w = item % sh.blocks2;
+    const uint32_t slot = a.first + g;
+    const brp_params& p = a.params[slot];
+    const uint32_t C = sh.c, M = sh.m, N = sh.n;
+    const bool middle = w == C / (2 * S);  // the block that owns the self-paired column C/2
+    const uint32_t base = middle ? C / 2 : w * S;
+
+    // Pad mean from the pass-1 partial sums, fixed order.
+    double part = 0.0;
+    for (uint32_t b = t; b < sh.blocks1; b += T) part += a.partial[size_t(g) * sh.blocks1 + b];
+    red[t] = part;
+    __syncthreads();
+    for (int d = T / 2; d > 0; d >>= 1) {
+        if (t < d) red[t] += red[t + d];
+        __syncthreads();
+    }
+    const float mu = float(red[0] / double(p.steps));
+    if (w == 0 && t == 0) a.mean[slot] = mu;
+
+    auto column = [&](int s) -> uint32_t {
+        const uint32_t lower = base + uint32_t(s % S);
+        return s < S ? lower : (C - lower) % C;
+    };
+
+    const float2* Y = a.Y + size_t(g) * M;
+    cf v[E];
+#pragma unroll
+    for (int u = 0; u < E / R0; ++u) {
+        const int b = t + T * u;
+        const int s = b % SS, j = b / SS;
+        const uint32_t col = column(s);
+#pragma unroll
+        for (int q = 0; q < R0; ++q) v[u * R0 + q] = load(Y + size_t(j + q * (R / R0)) * C + col);
+    }
+    fft_stages<R, SS, T, 0>(v, lds, a.tw_r, t);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < E / RL; ++u) {
+        const int b = t + T * u;
+        const int s = b % SS, j = b / SS;
+#pragma unroll
+        for (int q = 0; q < RL; ++q) store(lds + lds_at<SS>(j + q * (R / RL), s), v[u * RL + q]);
+    }
+    __syncthreads();
+
+    float* P = a.P + size_t(g) * M;
+    float2* snapX = a.snap_X ? a.snap_X + size_t(slot) * (M + 1) : nullptr;
+    const float inv_n = 1.0f / float(N);
+    const uint32_t steps = p.steps;
+    const float pad_len = float(N - steps);
+    const float parity = (steps & 1u) ? -1.0f : 1.0f;  // W_N^{M steps} = (-1)^steps
+    const double inv_nd = 1.0 / double(N);
+    for (int pi = t; pi < R * S; pi += T) {
+        const int s = pi % S;
+        const uint32_t k1 = uint32_t(pi / S);
+        if (middle && s != 0) continue;
+        const uint32_t col = base + uint32_t(s);
+        const bool self = col == 0 || col == C / 2;
+        const uint32_t k1m = col ? (R - 1 - k1) : (R - k1) % R;
+        const uint32_t k = k1 * C + col;
+        const cf zk = load(lds + lds_at<SS>(int(k1), s));
+        const cf zm = load(lds + lds_at<SS>(int(k1m), s + S));
+        const cf W = twiddle2(a.twn_lo, a.twn_hi, k);
+        const cf Wm = cf{-W.x, W.y};  // W_N^{M-k} = -conj(W_N^k)
+        cf xk = scale(add(add(zk, conj(zm)), mul_negi(mul(sub(zk, conj(zm)), W))), 0.5f);
+        cf xm = scale(add(add(zm, conj(zk)), mul_negi(mul(sub(zm, conj(zk)), Wm))), 0.5f);
+        if (k == 0) {
+            xk.x += mu * pad_len;
+            xm.x += mu * (parity - 1.0f) * 0.5f;  // U[M] = ((-1)^steps - 1) / 2
+        } else {
+            // e = k steps mod N, exact: the product is below 2^53.
+            const uint64_t prod = uint64_t(k) * steps;
+            int64_t e = int64_t(prod) - int64_t(floor(double(prod) * inv_nd)) * int64_t(N);
+            if (e < 0) e += N;
+            if (e >= int64_t(N)) e -= N;
+            const cf we = twiddle2(a.twn_lo, a.twn_hi, uint32_t(e));
+            const float ck = half_cot(W.x, W.y), cm = half_cot(Wm.x, Wm.y);
+            const cf uk = mul(cf{we.x - 1.0f, we.y}, cf{0.5f, -0.5f * ck});
+            const cf wem = cf{parity * we.x, -parity * we.y};
+            const cf um = mul(cf{wem.x - 1.0f, wem.y}, cf{0.5f, -0.5f * cm});
+            xk = cf{__builtin_fmaf(mu, uk.x, xk.x), __builtin_fmaf(mu, uk.y, xk.y)};
+            xm = cf{__builtin_fmaf(mu, um.x, xm.x), __builtin_fmaf(mu, um.y, xm.y)};
+        }
+        P[k] = k == 0 ? 0.0f : (xk.x * xk.x + xk.y * xk.y) * inv_n;
+        if (!self) P[M - k] = (xm.x * xm.x + xm.y * xm.y) * inv_n;
+        if (snapX) {
+            store(snapX + k, xk);
+            if (!self || k == 0) store(snapX + (M - k), xm);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- harmonic --
+template <int T>
+__global__ __launch_bounds__(T) void k_harmonic(FusedArgs a, FusedShape sh, uint32_t tiles) {
+    __shared__ float sums[4][kTile + 8];
+    const int t = threadIdx.x;
+    const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t g = item / tiles, tile = item % tiles;
+    const uint32_t slot = a.first + g;
+    const brp_params& p = a.params[slot];
+    const float* P = a.P + size_t(g) * sh.m;
+    const int64_t i0 = int64_t(tile) * kTile;
+    const uint32_t W = p.window, H = p.harmonic, F = p.fundamental;
+    uint32_t* res = a.results + size_t(slot) * a.slot_words;
+    float* snap = a.snap_summed ? a.snap_summed + size_t(slot) * 5 * a.snap_f : nullptr;
+
+    for (int idx = t; idx < kTile + 8; idx += T) {
+        const int64_t i = i0 - 8 + idx;
+        if (i >= int64_t(W) && i < int64_t(H)) {
+            float s[5];
+            dyadic_sums(P, uint32_t(i), s);
+            sums[0][idx] = s[1]; sums[1][idx] = s[2]; sums[2][idx] = s[3]; sums[3][idx] = s[4];
+        } else {
+            sums[0][idx] = -INFINITY; sums[1][idx] = -INFINITY; sums[2][idx] = -INFINITY; sums[3][idx] = -INFINITY;
+        }
+    }
+    // Level 0: the power itself.
+    for (int jj = t; jj < kTile; jj += T) {
+        const uint32_t j = uint32_t(i0) + jj;
+        if (j < F) {
+            const float v = P[j];
+            if (snap) snap[j] = v;
+            if (j >= W && v > p.threshold[0]) append_candidate(res, p.capacity, j, 0, v);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t h = 1; h <= 4; ++h) {
+        const int64_t width = int64_t(1) << h;
+        const uint32_t groups = kTile >> h;
+        const int64_t j0 = i0 >> h;
+        for (uint32_t gi = t; gi < groups; gi += T) {
+            const int64_t j = j0 + gi;
+            if (j >= int64_t(F)) continue;
+            const int64_t begin = max(int64_t(W), j * width - width / 2);
+            const int64_t end = min(int64_t(H), j * width + width / 2);
+            float value = 0.0f;
+            if (begin < end) {
+                // begin >= i0 - 8 and end <= i0 + kTile: inside the tile's sums.
+                const float* row = sums[h - 1];
+                const int lo = int(begin - i0 + 8), hi = int(end - i0 + 8);
+                const float first = row[lo];
+                float best = first;
+                for (int i = lo + 1; i < hi; ++i) best = fmaxf(best, row[i]);
+                value = best > p.threshold[h] ? best : first;
+            }
+            if (snap) snap[size_t(h) * a.snap_f + j] = value;
+            if (j >= int64_t(W) && value > p.threshold[h]) append_candidate(res, p.capacity, uint32_t(j), h, value);
+        }
+    }
+}
+
+__global__ void k_pack_results(const uint32_t* __restrict__ results, uint32_t slot_words, uint32_t nslots,
+                               uint32_t keep, uint32_t* __restrict__ staging) {
+    const uint32_t slot = blockIdx.x;
+    if (slot >= nslots) return;
+    const uint4* src = reinterpret_cast<const uint4*>(results + size_t(slot) * slot_words);
+    uint4* dst = reinterpret_cast<uint4*>(staging) + size_t(slot) * (keep + 1);
+    const uint32_t count = results[size_t(slot) * slot_words];
+    // The counter keeps counting past an overflow; never read beyond the slot's capacity.
+    const uint32_t n = min(min(count, keep), slot_words / 4 - 1);
+    for (uint32_t e = threadIdx.x; e <= n; e += blockDim.x) dst[e] = src[e];
+}
+
+template <int C, int S>
+void launch1(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
+    k_pass1<C, S, kThreads><<<sh.blocks1 * a.count, kThreads, 0, st>>>(a, sh);
+}
+template <int R, int S>
+void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
+    k_pass2<R, S, kThreads><<<sh.blocks2 * a.count, kThreads, 0, st>>>(a, sh);
+}
+
+struct ShapeRow { uint32_t m, c, r, s1, s2; };
+constexpr ShapeRow kShapes[] = {
+    {1u << 12, 64, 64, 64, 32},   {1u << 13, 128, 64, 32, 32},  {1u << 14, 128, 128, 32, 16},
+    {1u << 15, 256, 128, 16, 16}, {1u << 16, 256, 256, 16, 16}, {1u << 17, 512, 256, 16, 16},
+    {1u << 18, 1024, 256, 16, 16}, {1u << 19, 2048, 256, 8, 16}, {1u << 20, 4096, 256, 4, 16},
+    {1u << 21, 4096, 512, 4, 16}, {1u << 22, 4096, 1024, 4, 8},
+    {3u << 14, 384, 128, 32, 16}, {3u << 15, 768, 128, 16, 16}, {3u << 16, 768, 256, 16, 16},
+    {3u << 17, 1536, 256, 8, 16}, {3u << 18, 3072, 256, 4, 16}, {3u << 19, 3072, 512, 4, 16},
+    {3u << 20, 3072, 1024, 4, 8}, {3u << 21, 3072, 2048, 4, 4},
+};
+
+}  // namespace
+
+namespace brp_engine {
+
+bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* out) {
+    const uint32_t m = n / 2;
+    for (const ShapeRow& row : kShapes) {
+        if (row.m != m) continue;
+        if (c_override && c_override != row.c) return false;
+        FusedShape s;
+        s.n = n; s.m = m; s.c = row.c; s.r = row.r; s.s1 = row.s1; s.s2 = row.s2;
+        s.blocks1 = row.r / row.s1;
+        s.blocks2 = row.c / (2 * row.s2) + 1;
+        *out = s;
+        return true;
+    }
+    return false;
+}
+
+void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
+    switch (s.c * 1000 + s.s1) {
+        case 64 * 1000 + 64: launch1<64, 64>(st, s, a); break;
+        case 128 * 1000 + 32: launch1<128, 32>(st, s, a); break;
+        case 256 * 1000 + 16: launch1<256, 16>(st, s, a); break;
+        case 512 * 1000 + 16: launch1<512, 16>(st, s, a); break;
+        case 1024 * 1000 + 16: launch1<1024, 16>(st, s, a); break;
+        case 2048 * 1000 + 8: launch1<2048, 8>(st, s, a); break;
+        case 4096 * 1000 + 4: launch1<4096, 4>(st, s, a); break;
+        case 384 * 1000 + 32: launch1<384, 32>(st, s, a); break;
+        case 768 * 1000 + 16: launch1<768, 16>(st, s, a); break;
+        case 1536 * 1000 + 8: launch1<1536, 8>(st, s, a); break;
+        case 3072 * 1000 + 4: launch1<3072, 4>(st, s, a); break;
+        default: break;
+    }
+}
+
+void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
+    switch (s.r * 1000 + s.s2) {
+        case 64 * 1000 + 32: launch2<64, 32>(st, s, a); break;
+        case 128 * 1000 + 16: launch2<128, 16>(st, s, a); break;
+        case 256 * 1000 + 16: launch2<256, 16>(st, s, a); break;
+        case 512 * 1000 + 16: launch2<512, 16>(st, s, a); break;
+        case 1024 * 1000 + 8: launch2<1024, 8>(st, s, a); break;
+        case 2048 * 1000 + 4: launch2<2048, 4>(st, s, a); break;
+        default: break;
+    }
+}
+
+void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles) {
+    k_harmonic<kThreads><<<tiles * a.count, kThreads, 0, st>>>(a, s, tiles);
+}
+
+void pack_results(hipStream_t st, const uint32_t* results, uint32_t slot_words, uint32_t nslots, uint32_t keep,
+                  uint32_t* staging) {
+    k_pack_results<<<nslots, 64, 0, st>>>(results, slot_words, nslots, keep, staging);
+}
+
+}  // namespace brp_engine

@@ -1,0 +1,317 @@
+// science.cpp -- host-side science of the BRP4 per-template path: template parameters
+// (bench and bank semantics), the chi^2 thresholds and significances, the bank toplist,
+// the results file and the seeded synthetic workloads.
+//
+// Compiled with -ffp-contract=off: the FP32 expressions below must round exactly as
+// written, because resampling_steps() decides the resampled length with them
+// (reference CMakeLists.txt:14).  GPL-2.0-or-later.
+#include "brp_hip.h"
+#include "errors.h"
+#include "lut.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+// FP32 LUT sine with separately rounded operations (src/science.cpp:20-28).
+float lut_sin(float phase) {
+    float whole;
+    float frac = std::modf((1.0f / 6.283185f) * phase, &whole);
+    if (frac < 0) frac += 1;
+    const int seg = int(frac * 64.0f + 0.5f);
+    const float d = 6.283185f * (frac - (1.0f / 64) * float(seg));
+    float d2 = d;
+    d2 *= 0.5f * d;
+    const float s = brp_lut::kTable[2 * seg], c = brp_lut::kTable[2 * seg + 1];
+    return s + d * c - d2 * s;
+}
+
+// Nested-FMA variant used with flag 2 (src/science.cpp:29-35).
+float fma_sin(float phase) {
+    float whole;
+    float frac = std::modf((1.0f / 6.283185f) * phase, &whole);
+    if (frac < 0) frac += 1;
+    const int seg = int(std::fma(frac, 64.0f, 0.5f));
+    const float d = 6.283185f * std::fma(-float(seg), 1.0f / 64, frac);
+    const float d2 = (0.5f * d) * d;
+    const float s = brp_lut::kTable[2 * seg], c = brp_lut::kTable[2 * seg + 1];
+    return std::fma(-d2, s, std::fma(d, c, s));
+}
+
+// log of the upper tail of chi^2 with 2k degrees of freedom:
+// Q(x; 2k) = e^{-x/2} sum_{i<k} (x/2)^i / i!, summed in the log domain.
+double log_upper_tail(double x, uint32_t k) {
+    if (x <= 0) return 0.0;
+    const double half = 0.5 * x, lh = std::log(half);
+    double logs[64];
+    double top = -INFINITY, lfact = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        if (i) lfact += std::log(double(i));
+        logs[i] = double(i) * lh - lfact;
+        top = std::max(top, logs[i]);
+    }
+    double acc = 0;
+    for (uint32_t i = 0; i < k; ++i) acc += std::exp(logs[i] - top);
+    return top + std::log(acc) - half;
+}
+
+double upper_tail(double x, uint32_t k) {
+    const double l = log_upper_tail(x, k);
+    return l < -745.0 ? 0.0 : std::exp(l);
+}
+
+// Quantile of the upper tail: the x with Q(x; 2k) = q, by bisection on the monotone
+// log tail (the role gsl_cdf_chisq_Qinv plays at src/workload.cpp:122).
+double upper_tail_quantile(double q, uint32_t k) {
+    const double goal = std::log(q);
+    double lo = 0, hi = 1;
+    while (log_upper_tail(hi, k) > goal) hi *= 2;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        (log_upper_tail(mid, k) > goal ? lo : hi) = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+struct brp_toplist {
+    uint32_t rows = 100;
+    std::vector<brp_bank_row> table;  // 5 levels x rows, power-descending per level
+};
+
+extern "C" {
+
+float brp_lut_sin(float phase) { return lut_sin(phase); }
+
+float brp_sample_offset(uint32_t i, const brp_params* p) {
+    const float t = float(i) * p->dt;
+    if (p->flags & 2) return std::fma(p->tau * fma_sin(std::fma(p->omega, t, p->psi)), p->step_inv, -p->s0);
+    return p->tau * lut_sin(p->omega * t + p->psi) * p->step_inv - p->s0;
+}
+
+int brp_resampling_steps(brp_params* p) {
+    if (!p || p->unpadded < 2) return brp::fail(BRP_E_INVALID, "invalid template");
+    const float last = float(p->unpadded - 1);
+    uint32_t steps = p->unpadded - 1;
+    while (steps && float(steps) - brp_sample_offset(steps, p) >= last) --steps;
+    p->steps = steps;
+    if (!steps) return brp::fail(BRP_E_INVALID, "template produces empty series");
+    return BRP_OK;
+}
+
+int brp_parameters(uint32_t n, uint32_t unpadded, float tau, float period, float psi, float dt,
+                   brp_params* out) {
+    if (!out) return brp::fail(BRP_E_INVALID, "null output");
+    const uint32_t radix = is_pow2(n) ? 1 : ((n % 3 == 0 && is_pow2(n / 3)) ? 3 : 0);
+    if (!radix || n < 64 || n > 3u * (1u << 22) || unpadded < 32 || unpadded > n || dt <= 0 ||
+        period <= 0 || tau < 0 || !std::isfinite(tau + period + psi + dt))
+        return brp::fail(BRP_E_INVALID, "invalid observation/template");
+    brp_params p{};
+    p.n = n;
+    p.unpadded = unpadded;
+    p.fft_n = n;
+    p.fft_radix = radix;
+    p.fft_base = n / radix;
+    p.logn = uint32_t(__builtin_ctz(p.fft_base));
+    p.tau = tau;
+    p.omega = float(kTwoPi / double(period));
+    p.psi = psi;
+    p.dt = dt;
+    p.step_inv = 1.0f / dt;
+    // The historical driver takes S0 from a double sine, the resampler a LUT sine.
+    p.s0 = float(double(tau) * std::sin(double(psi)) * double(p.step_inv));
+    if (!std::isfinite(p.omega) || !std::isfinite(p.step_inv) || !std::isfinite(p.s0) ||
+        !std::isfinite(p.tau * p.step_inv) || !std::isfinite(float(unpadded) * p.dt) ||
+        !std::isfinite(p.omega * (float(unpadded) * p.dt) + p.psi))
+        return brp::fail(BRP_E_INVALID, "template exceeds finite FP32 arithmetic domain");
+    if (int rc = brp_resampling_steps(&p)) return rc;
+    p.harmonic = n / 2;
+    p.fundamental = n / 2;
+    p.window = 8;
+    const float thr[5] = {20, 30, 45, 70, 110};
+    std::copy(thr, thr + 5, p.threshold);
+    p.capacity = 65536;
+    *out = p;
+    return BRP_OK;
+}
+
+int brp_search_parameters(uint32_t n, uint32_t samples, double sample_us, const brp_orbit* orbit,
+                          double f0, double false_alarm, uint32_t window, brp_params* out) {
+    if (!orbit || !out) return brp::fail(BRP_E_INVALID, "null argument");
+    if (!std::isfinite(f0 + false_alarm) || f0 <= 0 || false_alarm <= 0 || false_alarm >= 1 || !window)
+        return brp::fail(BRP_E_INVALID, "invalid search parameters");
+    brp_params p{};
+    if (int rc = brp_parameters(n, samples, float(orbit->tau), float(orbit->period), float(orbit->psi),
+                                float(sample_us * 1e-6), &p))
+        return rc;
+    p.flags = 2 | 4;  // production FMA arithmetic
+    p.s0 = (p.tau * std::sin(p.psi)) * p.step_inv;  // single-precision sine here
+    if (int rc = brp_resampling_steps(&p)) return rc;
+    const double seconds = double(n) * sample_us * 1e-6;
+    p.window = uint32_t(window * 0.5 + 0.5);
+    if (p.window >= p.n / 2 + 1) return brp::fail(BRP_E_INVALID, "search window exceeds spectrum");
+    const double limit = double(p.n / 2 + 1 - p.window);
+    p.fundamental = uint32_t(std::min(limit, f0 * seconds + 0.5));
+    p.harmonic = uint32_t(std::min(limit, 16 * f0 * seconds + 0.5));
+    const float probability = float(1.0 - std::pow(1.0 - false_alarm, 1.0 / double(p.n / 2 + 1)));
+    if (!(probability > 0 && probability < 1))
+        return brp::fail(BRP_E_INVALID, "false-alarm probability outside representable domain");
+    for (uint32_t h = 0; h < 5; ++h)
+        p.threshold[h] = float(0.5 * upper_tail_quantile(double(probability), 1u << h));
+    if (p.fundamental <= p.window) return brp::fail(BRP_E_INVALID, "empty frequency search");
+    *out = p;
+    return BRP_OK;
+}
+
+// ---------------------------------------------------------------- toplist --
+int brp_toplist_create(uint32_t rows, brp_toplist** out) {
+    if (!out || !rows || rows > 100000) return brp::fail(BRP_E_INVALID, "invalid toplist size");
+    auto* t = new brp_toplist;
+    t->rows = rows;
+    t->table.assign(5 * size_t(rows), brp_bank_row{});
+    *out = t;
+    return BRP_OK;
+}
+
+void brp_toplist_destroy(brp_toplist* t) { delete t; }
+
+void brp_toplist_thresholds(const brp_toplist* t, brp_params* p) {
+    for (uint32_t h = 0; h < 5; ++h)
+        p->threshold[h] = std::max(p->threshold[h], float(t->table[(h + 1) * size_t(t->rows) - 1].power));
+}
+
+int brp_toplist_retain(brp_toplist* t, const brp_candidate* cand, size_t count, const brp_orbit* orbit,
+                       const brp_params* p) {
+    const uint32_t rows = t->rows;
+    for (size_t n = 0; n < count; ++n) {
+        const brp_candidate c = cand[n];
+        if (c.level >= 5 || c.bin < p->window || c.bin >= p->fundamental || !std::isfinite(c.power))
+            return brp::fail(BRP_E_INVALID, "invalid compact candidate");
+        brp_bank_row* level = t->table.data() + size_t(c.level) * rows;
+        brp_bank_row& lowest = level[rows - 1];
+        if (c.power <= p->threshold[c.level] || c.power <= lowest.power) continue;
+        // A bin already listed on this level is replaced only by a stronger detection.
+        brp_bank_row* slot = &lowest;
+        for (uint32_t i = 0; i < rows; ++i)
+            if (level[i].bin == c.bin) { slot = level[i].power < c.power ? &level[i] : nullptr; break; }
+        if (!slot) continue;
+        *slot = brp_bank_row{double(c.power), double(float(orbit->period)), double(float(orbit->tau)),
+                             double(float(orbit->psi)), 0.0, 1u << c.level, c.bin};
+        // glibc's qsort (the reference's) is a stable merge sort for this size.
+        std::stable_sort(level, level + rows,
+                         [](const brp_bank_row& a, const brp_bank_row& b) { return a.power > b.power; });
+    }
+    return BRP_OK;
+}
+
+size_t brp_toplist_final(const brp_toplist* t, brp_bank_row* out, size_t cap) {
+    std::vector<brp_bank_row> rows = t->table;
+    for (auto& r : rows) {
+        if (!r.harmonics) { r.significance = -10; continue; }
+        const double q = upper_tail(2 * r.power, r.harmonics);
+        r.significance = q > 0 ? -std::log10(q) : 320;
+        r.power /= double(float(std::sqrt(float(r.harmonics))));
+    }
+    auto by_sig = [](const brp_bank_row& a, const brp_bank_row& b) { return a.significance > b.significance; };
+    std::stable_sort(rows.begin(), rows.end(), by_sig);
+    size_t n = 0;
+    while (n < t->rows && rows[0].significance > 0) {
+        if (n < cap) out[n] = rows[0];
+        ++n;
+        const uint32_t bin = rows[0].bin;
+        for (auto& r : rows) if (r.bin == bin) r.significance = -10;
+        std::stable_sort(rows.begin(), rows.end(), by_sig);
+    }
+    return n;
+}
+
+size_t brp_toplist_rows(const brp_toplist* t, brp_bank_row* out, size_t cap) {
+    const size_t n = t->table.size();
+    if (out) std::copy_n(t->table.begin(), std::min(n, cap), out);
+    return n;
+}
+
+int brp_write_candidates(const char* path, const brp_bank_row* rows, size_t count, double seconds) {
+    std::FILE* f = std::fopen(path, "w");
+    if (!f) return brp::fail(BRP_E_RUNTIME, "cannot write offline candidates");
+    std::fprintf(f, "%% Experimental offline BRP4 results; never submit to production\n");
+    for (size_t i = 0; i < count; ++i) {
+        const brp_bank_row& c = rows[i];
+        std::fprintf(f, "%6.12f %6.12f %6.12f %6.12f %g %g %d\n", c.bin / seconds, c.period, c.tau, c.psi,
+                     c.power, c.significance, int(c.harmonics));
+    }
+    std::fprintf(f, "%%DONE%%\n");
+    if (std::fclose(f)) return brp::fail(BRP_E_RUNTIME, "candidate write failed");
+    return BRP_OK;
+}
+
+// -------------------------------------------------------------- synthetic --
+double brp_uniform(uint64_t seed, uint64_t counter) {
+    const uint64_t z = mix64(mix64(seed + 0x9E3779B97F4A7C15ull) + counter * 0xD1B54A32D192ED03ull);
+    return double(z >> 11) * 0x1.0p-53;
+}
+
+int brp_synthetic_series(uint32_t count, double dt, uint64_t seed, double mean, double sigma,
+                         double spin_hz, double duty, double amplitude, const brp_orbit* orbit,
+                         float* out) {
+    if (!out || !(dt > 0) || !(sigma >= 0)) return brp::fail(BRP_E_INVALID, "invalid synthetic series");
+    const bool pulsed = amplitude != 0 && spin_hz > 0;
+    double omega = 0, tau = 0, psi = 0;
+    if (pulsed && orbit) {
+        omega = kTwoPi / orbit->period;
+        tau = orbit->tau;
+        psi = orbit->psi;
+    }
+    const double sin_psi = std::sin(psi);
+    for (uint32_t j = 0; j < count; j += 2) {
+        const uint64_t pair = j / 2;
+        const double u1 = brp_uniform(seed, 2 * pair), u2 = brp_uniform(seed, 2 * pair + 1);
+        const double radius = std::sqrt(-2.0 * std::log(1.0 - u1));
+        const double g[2] = {radius * std::cos(kTwoPi * u2), radius * std::sin(kTwoPi * u2)};
+        for (uint32_t q = 0; q < 2 && j + q < count; ++q) {
+            double v = mean + sigma * g[q];
+            if (pulsed) {
+                const double t_obs = double(j + q) * dt;
+                double tp = t_obs;  // fixed point of tp = t_obs + tau (sin(omega tp + psi) - sin psi)
+                for (int it = 0; it < 8; ++it) tp = t_obs + tau * (std::sin(omega * tp + psi) - sin_psi);
+                const double phase = spin_hz * tp - std::floor(spin_hz * tp);
+                if (phase < duty) v += amplitude * sigma;
+            }
+            out[j + q] = float(v);
+        }
+    }
+    return BRP_OK;
+}
+
+int brp_synthetic_bank(uint64_t seed, uint32_t count, double pmin, double pmax, double tau_max,
+                       brp_orbit* out) {
+    if (!out || !(pmin > 0) || !(pmax >= pmin) || !(tau_max >= 0))
+        return brp::fail(BRP_E_INVALID, "invalid synthetic bank");
+    const double lmin = std::log(pmin), lmax = std::log(pmax);
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint64_t c = 3ull * k;
+        out[k].period = std::exp(lmin + brp_uniform(seed, c) * (lmax - lmin));
+        out[k].tau = brp_uniform(seed, c + 1) * tau_max;
+        out[k].psi = brp_uniform(seed, c + 2) * kTwoPi;
+    }
+    return BRP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+/*
+ * brp_hip.h -- C ABI of the MI355X (gfx950) BRP4 per-template search backend.
+ *
+ * This is the drop-in boundary.  It replaces the reference's Vulkan backend
+ * class brp::GPU (include/brp/vulkan.hpp:6-29, src/vulkan.cpp:278-406) and the
+ * host science helpers that feed it (include/brp/science.hpp:8-38,
+ * include/brp/workload.hpp:6-24).  Plain pointers and sizes only; no C++ or torch
+ * types cross it.  include/brp/hip.hpp wraps it in the reference's C++ class shape.
+ *
+ * Errors: every function returning int returns 0 on success and a nonzero
+ * BRP_E_* class otherwise; brp_last_error() then holds a message for the calling
+ * thread (the reference threw std::runtime_error with the same messages).
+ *
+ * Threading: one handle per host thread; calls on one handle must not overlap.
+ */
+#ifndef BRP_HIP_H
+#define BRP_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- types ---- */
+
+/* Per-template parameter block, byte-compatible with the reference Params
+ * (include/brp/science.hpp:8-17, 108 bytes).  Flag bits (src/vulkan.cpp:111,
+ * 279-282): 1 packed real FFT, 2 production FMA arithmetic in the resampler,
+ * 4 V3D FMA emulation (exact FMA is native here, so 4 changes nothing),
+ * 8 scatter harmonic search, 16 radix-8 stages, 32 radix-8 paired store,
+ * 64 fused power. */
+typedef struct brp_params {
+    uint32_t n, unpadded, steps, logn;
+    float tau, omega, psi, dt, step_inv, s0;
+    uint32_t fundamental, harmonic, window, stage;
+    float threshold[5];
+    uint32_t capacity;
+    uint32_t fft_n, fft_base, fft_radix, flags, twiddle3, twiddle_real;
+    uint32_t result_slot;
+} brp_params;
+
+/* include/brp/science.hpp:18 */
+typedef struct brp_candidate { uint32_t bin, level; float power; uint32_t reserved; } brp_candidate;
+
+/* Orbit of the bank (include/brp/workload.hpp:13). */
+typedef struct brp_orbit { double period, tau, psi; } brp_orbit;
+
+/* Toplist row (include/brp/workload.hpp:15). */
+typedef struct brp_bank_row {
+    double power, period, tau, psi, significance;
+    uint32_t harmonics, bin;
+} brp_bank_row;
+
+/* Engines. */
+enum {
+    BRP_ENGINE_PLAIN = 0, /* reference-shaped: one template per launch sequence,
+                             one kernel per stage (resample, mean, pack, Stockham
+                             passes, real split, power, harmonic) */
+    BRP_ENGINE_FUSED = 1  /* two-pass four-step FFT with resampling fused into pass 1
+                             and real split + power fused into pass 2; all templates
+                             of a batch in each launch; hipGraph replay */
+};
+
+/* Constructor options.  The first eleven fields are the reference GPU constructor's
+ * arguments (include/brp/vulkan.hpp:14) with the same meaning and validation
+ * (src/vulkan.cpp:116-122, 279-283). */
+typedef struct brp_options {
+    uint32_t workgroup;    /* 16..256; plain engine block size */
+    uint32_t tile;         /* 0 or 64..4096 power of two (reference LDS DIT tile); accepted,
+                              validated, and mapped onto Stockham stages */
+    uint32_t batch;        /* templates per run (result slots); 1..4096 */
+    uint32_t software;     /* accepted for API compatibility; no effect */
+    uint32_t stockham, radix4, packed, scatter, radix8, fused_power, radix8_store;
+    uint32_t engine;       /* BRP_ENGINE_* */
+    uint32_t graph;        /* fused engine: replay the batch as a captured hipGraph */
+    uint32_t snapshot;     /* keep resampled / FFT / summed arrays for snapshot() */
+    uint32_t fp64_phase;   /* evaluate the orbital offset in FP64 (variant; indices may
+                              differ from the FP32 LUT rule, which is the parity mode) */
+    uint32_t split_c;      /* fused engine: pass-1 FFT length C (0 = default table) */
+    uint32_t precision;    /* 0 = FP32 FFT; 1 = split-FP16 MFMA first radix-16 stages */
+    uint32_t group;        /* fused engine: templates per launch (0 = 8) */
+    uint32_t reserved[5];
+} brp_options;
+
+/* src/vulkan.hpp:6 Timings; template_stage_ms is returned separately. */
+typedef struct brp_timings {
+    double wall_ms, cpu_ms, submit_ms;
+    double stage_ms[4]; /* resample_prepare, fft, power, harmonic_candidates */
+} brp_timings;
+
+typedef struct brp_hip brp_hip;
+
+enum { BRP_OK = 0, BRP_E_INVALID = 1, BRP_E_RUNTIME = 2, BRP_E_OVERFLOW = 3, BRP_E_DEVICE = 4 };
+
+const char* brp_last_error(void);
+
+/* ------------------------------------------------------- host science ------- */
+
+/* src/science.cpp:20-28 */
+float brp_lut_sin(float phase);
+/* src/science.cpp:36-40 */
+float brp_sample_offset(uint32_t i, const brp_params* p);
+/* src/science.cpp:41-45 */
+int brp_resampling_steps(brp_params* p);
+/* src/science.cpp:46-65 (bench semantics) */
+int brp_parameters(uint32_t n, uint32_t unpadded, float tau, float period, float psi, float dt,
+                   brp_params* out);
+/* src/workload.cpp:108-125 (bank semantics); the chi^2 quantile uses the closed form of
+ * the even-dof upper tail in place of GSL's gsl_cdf_chisq_Qinv. */
+int brp_search_parameters(uint32_t n, uint32_t samples, double sample_us, const brp_orbit* orbit,
+                          double f0, double false_alarm, uint32_t window, brp_params* out);
+
+/* Toplist (src/workload.cpp:126-159), rows per level configurable (reference: 100). */
+typedef struct brp_toplist brp_toplist;
+int brp_toplist_create(uint32_t rows_per_level, brp_toplist** out);
+void brp_toplist_destroy(brp_toplist* t);
+void brp_toplist_thresholds(const brp_toplist* t, brp_params* p);
+int brp_toplist_retain(brp_toplist* t, const brp_candidate* c, size_t count, const brp_orbit* orbit,
+                       const brp_params* p);
+/* Final significance-ranked rows; returns the row count (written up to cap). */
+size_t brp_toplist_final(const brp_toplist* t, brp_bank_row* out, size_t cap);
+/* Raw 5 x rows table (level-major, power-descending within a level). */
+size_t brp_toplist_rows(const brp_toplist* t, brp_bank_row* out, size_t cap);
+/* src/workload.cpp:160-165 results file ("%DONE%" terminated). */
+int brp_write_candidates(const char* path, const brp_bank_row* rows, size_t count, double seconds);
+
+/* ----------------------------------------------- synthetic workloads -------- */
+/* Seeded N(0, sigma^2) noise (counter-based RNG + Box-Muller in double) plus a
+ * rectangular pulse train (frequency, duty, amplitude in units of sigma) whose pulse
+ * times are Doppler-shifted by a circular orbit: pulsar-frame time t_p of sample j
+ * solves t_p - tau sin(omega t_p + psi) + tau sin psi = j dt.  amplitude 0 gives pure
+ * noise.  Matches the resampler's model, so the matching template makes the train
+ * periodic. */
+int brp_synthetic_series(uint32_t count, double dt, uint64_t seed, double mean, double sigma,
+                         double spin_hz, double duty, double amplitude, const brp_orbit* orbit,
+                         float* out);
+/* Orbits drawn from a seed: P log-uniform in [pmin, pmax], tau uniform in [0, tau_max],
+ * psi uniform in [0, 2 pi). */
+int brp_synthetic_bank(uint64_t seed, uint32_t count, double pmin, double pmax, double tau_max,
+                       brp_orbit* out);
+/* Counter-based uniform in [0, 1) (SplitMix64 of seed and counter). */
+double brp_uniform(uint64_t seed, uint64_t counter);
+
+/* ------------------------------------------------------------ device -------- */
+
+/* GPU(const Params&, span<const float> input, ...) -- include/brp/vulkan.hpp:14.
+ * Copies `count` == p->unpadded input samples to the device. */
+int brp_hip_create(const brp_params* p, const float* input, size_t count, const brp_options* opt,
+                   brp_hip** out);
+int brp_hip_destroy(brp_hip* h);
+/* set_templates -- src/vulkan.cpp:289-295.  count <= options.batch.  Templates must keep
+ * n, unpadded, fundamental, harmonic and capacity, and the arithmetic flags (2|4). */
+int brp_hip_set_templates(brp_hip* h, const brp_params* p, size_t count);
+/* run -- src/vulkan.cpp:296-318.  Blocks until the batch is done.  template_stage_ms
+ * (may be NULL) receives 4 doubles per template. */
+int brp_hip_run(brp_hip* h, brp_timings* t, double* template_stage_ms);
+/* candidates(slot) -- src/vulkan.cpp:329-337: sorted by (level, bin).  Returns
+ * BRP_E_OVERFLOW on candidate overflow or an out-of-range resample index.  *count gets
+ * the number of candidates even when cap is too small (then nothing beyond cap is
+ * written and BRP_E_INVALID is returned). */
+int brp_hip_candidates(brp_hip* h, uint32_t slot, brp_candidate* out, size_t cap, size_t* count);
+/* snapshot() -- src/vulkan.cpp:319-328, for the given slot (needs options.snapshot for the
+ * fused engine).  Any pointer may be NULL.  resampled: n floats; index: steps int32
+ * (resample source index); fft: 2*(n/2+1) floats; power: n/2 floats; summed:
+ * 5*fundamental floats. */
+int brp_hip_snapshot(brp_hip* h, uint32_t slot, float* resampled, int32_t* index, float* fft,
+                     float* power, float* summed);
+/* device_info / allocated_bytes / profile_counters -- src/vulkan.cpp:338-406. */
+size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap);
+uint64_t brp_hip_allocated_bytes(const brp_hip* h);
+size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap);
+/* Per-kernel device times of the last run (ms), summed over the batch, as measured
+ * with HIP events on the launch stream: pass1/resample, fft, power, harmonic. */
+int brp_hip_kernel_ms(brp_hip* h, double out[4]);
+
+/* Bank search: the brp_workunit bank loop (src/workunit_main.cpp:77-136) for a bank of
+ * `count` orbits against the handle's series, in batches of options.batch templates,
+ * with thresholds frozen at each batch start and Toplist::retain replayed in template
+ * order, so the toplist equals the one-template-at-a-time result.  Template parameters
+ * come from brp_search_parameters(n, samples, sample_us, ...).  templates_out (may be
+ * NULL) receives the count parameter blocks as run; cand_count (may be NULL) the
+ * candidate count of each template. */
+int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint32_t samples,
+                        double sample_us, double f0, double false_alarm, uint32_t window,
+                        brp_toplist* top, brp_params* templates_out, uint32_t* cand_count,
+                        double* gpu_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,207 @@
+"""GPU parity of both engines against the oracle, through the C ABI (libbrp_hip.so).
+
+Every case prints its name before touching the GPU.  Sizes: the N = 65536 and
+3 * 2^15 cases run the whole oracle; the N = 2^22 and 3 * 2^20 cases compare against
+the oracle at full size (its FP64 FFT takes about a second there).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import parity
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def wl():
+    import workloads
+
+    return workloads
+
+
+PLAIN_VARIANTS = {
+    # the reference's selected configuration: packed, Stockham, radix-8 store, fused power, scatter
+    "selected": dict(stockham=True, packed=True, radix8=True, radix8_store=True, fused_power=True, scatter=True),
+    "radix4_gather": dict(stockham=True, packed=True, radix4=True),
+    "full_complex_radix2": dict(stockham=True),
+}
+
+
+@pytest.mark.parametrize("variant", sorted(PLAIN_VARIANTS))
+@pytest.mark.parametrize("scale", [1.0, 0.25])
+def test_plain_small(brp, oracle, wl, variant, scale):
+    w = wl.plain_small(threshold_scale=scale)
+    p = w.templates[0]
+    say(f"[gpu] plain N=65536 variant={variant} scale={scale}")
+    with brp.GPU(p, w.x, 256, 0, 1, snapshot=True, **PLAIN_VARIANTS[variant]) as g:
+        g.set_template(p)
+        g.run()
+        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"plain/{variant}")
+    say(rep)
+    if scale < 1:
+        assert rep["candidates"]["ref"] > 0
+
+
+@pytest.mark.parametrize("scale", [1.0, 0.25])
+@pytest.mark.parametrize("flags", [0, 6])
+def test_fused_small(brp, oracle, wl, scale, flags):
+    w = wl.plain_small(threshold_scale=scale, flags=flags)
+    p = w.templates[0]
+    say(f"[gpu] fused N=65536 scale={scale} flags={flags}")
+    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True) as g:
+        g.set_template(p)
+        g.run()
+        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, "fused/small")
+    say(rep)
+
+
+@pytest.mark.parametrize("n", [3 << 15, 1 << 17, 3 << 17])
+def test_fused_lengths(brp, oracle, wl, n):
+    w = wl.plain_full(seed=5, n=n, threshold_scale=0.3)
+    p = w.templates[0]
+    say(f"[gpu] fused N={n}")
+    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True) as g:
+        g.set_template(p)
+        g.run()
+        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"fused/N={n}")
+    say(rep)
+
+
+def test_plain_mixed_radix_small(brp, oracle, wl):
+    w = wl.plain_full(seed=6, n=3 << 15, threshold_scale=0.3)
+    p = w.templates[0]
+    say("[gpu] plain N=3*2^15 selected")
+    with brp.GPU(p, w.x, 256, 0, 1, snapshot=True, **PLAIN_VARIANTS["selected"]) as g:
+        g.set_template(p)
+        g.run()
+        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, "plain/3*2^15")
+    say(rep)
+
+
+@pytest.mark.parametrize("engine", ["fused", "plain"])
+def test_full_size_2p22(brp, oracle, wl, engine):
+    w = wl.plain_full(seed=42)
+    p = w.templates[0]
+    say(f"[gpu] {engine} N=2^22 spin={w.spin_hz:.3f} Hz")
+    kw = dict(packed=True, engine=brp.ENGINE_FUSED) if engine == "fused" else PLAIN_VARIANTS["selected"]
+    with brp.GPU(p, w.x, 256, 0, 1, snapshot=True, **kw) as g:
+        g.set_template(p)
+        g.run()
+        snap = g.snapshot(0)
+        rep = parity.check_template(snap, oracle, w.x, p, f"{engine}/2^22")
+        # The injected pulsar is found by the matching template.
+        assert len(snap.candidates) > 0
+        g.run()
+        again = g.snapshot(0)
+        assert np.array_equal(snap.power, again.power) and np.array_equal(snap.summed, again.summed)
+    say(rep)
+
+
+def test_full_size_mixed_radix(brp, oracle, wl):
+    w = wl.mixed_radix(count=3)
+    say("[gpu] fused N=3*2^20, 3 templates in one launch")
+    with brp.GPU(w.templates[0], w.x, 256, 0, 3, packed=True, engine=brp.ENGINE_FUSED, snapshot=True) as g:
+        g.set_templates(w.templates)
+        g.run()
+        for s, p in enumerate(w.templates):
+            rep = parity.check_template(g.snapshot(s), oracle, w.x, p, f"fused/3*2^20/slot{s}")
+            say(rep)
+
+
+def test_fused_batch_slots_match_single_runs(brp, wl):
+    w = wl.bank(count=6, n=1 << 20)
+    say("[gpu] fused batch of 6 (group 4, graph) vs single-template runs, N=2^20")
+    with brp.GPU(w.templates[0], w.x, 256, 0, 6, packed=True, engine=brp.ENGINE_FUSED, graph=True, group=4) as g:
+        g.set_templates(w.templates)
+        g.run()
+        batched = [g.candidates(s) for s in range(6)]
+        g.run()
+        assert all(np.array_equal(a, g.candidates(s)) for s, a in enumerate(batched))
+    with brp.GPU(w.templates[0], w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED) as g:
+        for s, p in enumerate(w.templates):
+            g.set_template(p)
+            g.run()
+            assert np.array_equal(g.candidates(0), batched[s])
+
+
+def _oracle_bank(oracle, x, templates, orbits, rows=100):
+    top = oracle.OToplist(rows)
+    for p, o in zip(templates, orbits):
+        q = oracle.as_oparams(p)
+        top.thresholds(q)
+        res, idx = oracle.resample(x, q)
+        p32, _ = oracle.power(res)
+        cands = oracle.select(oracle.harmonic(p32, q), q)
+        top.retain(cands, o, q)
+    return top
+
+
+def test_bank_toplist_identical(brp, oracle, wl):
+    count, n = 24, 1 << 20
+    w = wl.bank(count=count, n=n)
+    say(f"[gpu] bank {count} x N={n}: fused batched (batch 8) vs plain batch 1 vs oracle loop")
+    tops = {}
+    for name, kw in {"fused": dict(packed=True, engine=brp.ENGINE_FUSED, graph=True, batch=8),
+                     "plain": dict(engine=brp.ENGINE_PLAIN, batch=1, **PLAIN_VARIANTS["selected"])}.items():
+        batch = kw.pop("batch")
+        top = brp.Toplist(100)
+        with brp.GPU(w.templates[0], w.x, 256, 0, batch, **kw) as g:
+            g.bank_search(w.orbits, n, 64.0, top)
+        tops[name] = top
+    ref = _oracle_bank(oracle, w.x, w.templates, w.orbits)
+    refrows = [(r.bin, r.harmonics, r.period) for r in ref.state]
+    for name, top in tops.items():
+        rows = [(r.bin, r.harmonics, r.period) for r in top.table()]
+        assert rows == refrows, name
+        pw = np.array([r.power for r in top.table()])
+        rp = np.array([r.power for r in ref.state])
+        assert np.allclose(pw, rp, rtol=1e-5, atol=0), name
+        assert [r.bin for r in top.final()] == [r.bin for r in ref.final()], name
+    assert sum(r.harmonics > 0 for r in ref.state) > 0
+
+
+def test_candidate_overflow_raises(brp, wl):
+    w = wl.plain_small(threshold_scale=0.2)
+    p = w.templates[0]
+    p.capacity = 4
+    say("[gpu] overflow: capacity 4 with low thresholds")
+    for kw in (dict(packed=True, engine=brp.ENGINE_FUSED), dict(PLAIN_VARIANTS["selected"])):
+        with brp.GPU(p, w.x, 256, 0, 1, **kw) as g:
+            g.set_template(p)
+            g.run()
+            with pytest.raises(RuntimeError, match="overflow"):
+                g.candidates(0)
+
+
+def test_out_of_range_index_flagged(brp, wl):
+    w = wl.plain_small()
+    p = w.templates[0]
+    # Keep every sample although the last ones map past the end of the observation.
+    p.steps = p.unpadded
+    say("[gpu] resample index past the observation is flagged")
+    for kw in (dict(packed=True, engine=brp.ENGINE_FUSED), dict(PLAIN_VARIANTS["selected"])):
+        with brp.GPU(p, w.x, 256, 0, 1, **kw) as g:
+            g.set_template(p)
+            g.run()
+            with pytest.raises(RuntimeError, match="invalid resample index"):
+                g.candidates(0)
+
+
+def test_template_dimension_change_rejected(brp, wl):
+    w = wl.plain_small()
+    p = w.templates[0]
+    q = p.copy()
+    q.fundamental -= 1
+    say("[gpu] set_templates rejects a dimension change")
+    with brp.GPU(p, w.x, 256, 0, 2, packed=True, engine=brp.ENGINE_FUSED) as g:
+        with pytest.raises(RuntimeError, match="dimensions changed"):
+            g.set_templates([p, q])
+        with pytest.raises(RuntimeError, match="batch exceeds"):
+            g.set_templates([p, p, p])
