@@ -189,7 +189,11 @@ def uniform(seed: int, counter: int) -> float:
 
 
 def candidates_array(c) -> np.ndarray:
-    return np.frombuffer(bytes(c), dtype=CANDIDATE_DTYPE).copy() if len(c) else np.zeros(0, CANDIDATE_DTYPE)
+    """Structured array (bin, level, power, reserved) from a sequence of Candidate."""
+    out = np.zeros(len(c), CANDIDATE_DTYPE)
+    for i, x in enumerate(c):
+        out[i] = (x.bin, x.level, x.power, x.reserved)
+    return out
 
 
 # ----------------------------------------------------------------- toplist --
@@ -306,7 +310,7 @@ class GPU:
         _check(lib.brp_hip_candidates(self._h, slot, None, 0, ctypes.byref(n)))
         out = (Candidate * max(n.value, 1))()
         _check(lib.brp_hip_candidates(self._h, slot, out, n.value, ctypes.byref(n)))
-        return candidates_array(out[:n.value] if n.value else [])
+        return np.frombuffer(out, dtype=CANDIDATE_DTYPE)[:n.value].copy()
 
     def snapshot(self, slot: int | None = None, p: Params | None = None) -> Snapshot:
         if slot is None:

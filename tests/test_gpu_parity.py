@@ -182,8 +182,10 @@ def test_candidate_overflow_raises(brp, wl):
 
 def test_out_of_range_index_flagged(brp, wl):
     w = wl.plain_small()
-    p = w.templates[0]
-    # Keep every sample although the last ones map past the end of the observation.
+    # psi = 2.5: the orbital offset falls by ~4.6 samples over the series, so the last
+    # samples map past its end.  Keep every sample anyway (steps = unpadded).
+    p = brp.parameters(w.n, w.n, 0.05, 3600.0, 2.5, wl.DT)
+    assert brp.sample_offset(w.n - 1, p) < -1.0
     p.steps = p.unpadded
     say("[gpu] resample index past the observation is flagged")
     for kw in (dict(packed=True, engine=brp.ENGINE_FUSED), dict(PLAIN_VARIANTS["selected"])):
