@@ -67,6 +67,11 @@ def main() -> int:
     H = w.templates[0].harmonic
     log(f"[bench] workload {w.name}, spin {w.spin_hz:.3f} Hz, built in {time.time() - t0:.1f} s")
 
+    # Plain streaming copy of a known size: the achievable bandwidth on this box, and the
+    # calibration kernel (brp_copy) for the hardware byte counters of profiled runs.
+    copy_gbps = brp.copy_bandwidth(256 << 20, 20)
+    log(f"[bench] streaming copy of 256 MiB: {copy_gbps:.1f} GB/s")
+
     # ---------------------------------------------------------------- batched --
     g = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
                 graph=not args.no_graph, group=args.group)
@@ -101,22 +106,22 @@ def main() -> int:
 
     # ------------------------------------------------------------------ plain --
     plain_n = min(args.plain_templates, args.templates)
-    gp = brp.GPU(w.templates[0], w.x, 256, 0, 1, False, True, False, True, True, True, True, True,
-                 engine=brp.ENGINE_PLAIN)
-    top_p = brp.Toplist(100)
-    gp.bank_search(w.orbits[:1], U, workloads.SAMPLE_US, brp.Toplist(100))  # warm-up
-    tp0 = time.perf_counter()
-    _, _, plain_gpu_ms = gp.bank_search(w.orbits[:plain_n], U, workloads.SAMPLE_US, top_p)
-    plain_elapsed = time.perf_counter() - tp0
-    plain_tps = plain_n / plain_elapsed
-    gp.close()
-    identical = None
+    plain_tps, identical = None, None
+    if plain_n > 0:
+        gp = brp.GPU(w.templates[0], w.x, 256, 0, 1, False, True, False, True, True, True, True, True,
+                     engine=brp.ENGINE_PLAIN)
+        top_p = brp.Toplist(100)
+        gp.bank_search(w.orbits[:1], U, workloads.SAMPLE_US, brp.Toplist(100))  # warm-up
+        tp0 = time.perf_counter()
+        gp.bank_search(w.orbits[:plain_n], U, workloads.SAMPLE_US, top_p)
+        plain_tps = plain_n / (time.perf_counter() - tp0)
+        gp.close()
     if plain_n == args.templates:
         prow = [(r.bin, r.harmonics, r.period, r.power) for r in top_p.table()]
         identical = [r[:3] for r in prow] == [r[:3] for r in batched_rows] and \
             bool(np.allclose([r[3] for r in prow], [r[3] for r in batched_rows], rtol=1e-5, atol=0)) and \
             [r.bin for r in top_p.final()] == final_bins
-    log(f"[bench] plain: {plain_tps:.1f} templates/s over {plain_n}; toplist identical: {identical}")
+    log(f"[bench] plain: {plain_tps} templates/s over {plain_n}; toplist identical: {identical}")
 
     # --------------------------------------------------------------- roofline --
     B = workloads.algorithmic_bytes(n, U, H)
@@ -166,8 +171,8 @@ def main() -> int:
         "config": {"workload": f"bank of {args.templates} orbital templates x N={n} series (bank semantics)",
                    "templates": args.templates, "n": n, "batch": args.batch, "group": args.group,
                    "graph": not args.no_graph},
-        "plain_templates_per_s": round(plain_tps, 3),
-        "batched_over_plain": round(templates_per_s / plain_tps, 3),
+        "plain_templates_per_s": round(plain_tps, 3) if plain_tps else None,
+        "batched_over_plain": round(templates_per_s / plain_tps, 3) if plain_tps else None,
         "toplist_identical": identical,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
@@ -176,6 +181,7 @@ def main() -> int:
         "pipeline_roofline": {"bytes_per_template": B, "achieved_GBps": round(pipeline_achieved / 1e9, 2),
                               "frac": round(pipeline_achieved / HBM_PEAK, 4)},
         "kernel_ms_last_run": {k: round(v, 4) for k, v in kms.items()},
+        "copy_GBps": round(copy_gbps, 1),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -118,6 +118,7 @@ def load() -> ctypes.CDLL:
         "brp_hip_allocated_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
         "brp_hip_profile_counters": (sz, [ctypes.c_void_p, ctypes.c_char_p, sz]),
         "brp_hip_kernel_ms": (i32, [ctypes.c_void_p, P(ctypes.c_double)]),
+        "brp_hip_copy_bandwidth": (i32, [sz, i32, P(ctypes.c_double)]),
         "brp_hip_bank_search": (i32, [ctypes.c_void_p, P(Orbit), sz, u32, f64, f64, f64, u32, ctypes.c_void_p,
                                       P(Params), P(u32), P(ctypes.c_double)]),
     }
@@ -182,6 +183,13 @@ def synthetic_bank(seed: int, count: int, pmin: float = 660.0, pmax: float = 100
     arr = (Orbit * count)()
     _check(load().brp_synthetic_bank(seed, count, pmin, pmax, tau_max, arr))
     return arr
+
+
+def copy_bandwidth(nbytes: int, reps: int = 10) -> float:
+    """GB/s (read + write) of a plain device-to-device streaming copy of nbytes."""
+    out = ctypes.c_double()
+    _check(load().brp_hip_copy_bandwidth(nbytes, reps, ctypes.byref(out)))
+    return out.value
 
 
 def uniform(seed: int, counter: int) -> float:

@@ -70,11 +70,41 @@ __device__ __forceinline__ int source_index(uint32_t i, const Orbit32& o, const 
     return int(double(idx) + 0.5);
 }
 
-// FP64 variant: the orbital offset from the exact sine in double precision.
+// sin(x) in FP64 for orbital phases (|x| well below 2^30): Cody-Waite reduction by
+// multiples of pi/2 (pi/2 split into a double head and tail) to |r| <= pi/4, then the
+// Taylor series of sin / cos through r^17 / r^16 in Horner form; the first omitted term
+// is below 1e-17 on that interval.
+__device__ __forceinline__ double sin_f64(double x) {
+    const double k = __builtin_rint(x * 0.63661977236758134308);
+    const double r = __builtin_fma(-k, 6.123233995736766036e-17, __builtin_fma(-k, 1.5707963267948966192, x));
+    const double r2 = r * r;
+    double s = 2.8114572543455207632e-15;                 //  1/17!
+    s = __builtin_fma(s, r2, -7.6471637318198164759e-13);  // -1/15!
+    s = __builtin_fma(s, r2, 1.6059043836821614599e-10);   //  1/13!
+    s = __builtin_fma(s, r2, -2.5052108385441718775e-8);   // -1/11!
+    s = __builtin_fma(s, r2, 2.7557319223985890653e-6);    //  1/9!
+    s = __builtin_fma(s, r2, -1.9841269841269841270e-4);   // -1/7!
+    s = __builtin_fma(s, r2, 8.3333333333333333333e-3);    //  1/5!
+    s = __builtin_fma(s, r2, -1.6666666666666666667e-1);   // -1/3!
+    s = __builtin_fma(s * r2, r, r);
+    double c = 4.7794773323873852974e-14;                 //  1/16!
+    c = __builtin_fma(c, r2, -1.1470745597729724714e-11);  // -1/14!
+    c = __builtin_fma(c, r2, 2.0876756987868098979e-9);    //  1/12!
+    c = __builtin_fma(c, r2, -2.7557319223985890653e-7);   // -1/10!
+    c = __builtin_fma(c, r2, 2.4801587301587301587e-5);    //  1/8!
+    c = __builtin_fma(c, r2, -1.3888888888888888889e-3);   // -1/6!
+    c = __builtin_fma(c, r2, 4.1666666666666666667e-2);    //  1/4!
+    c = __builtin_fma(c, r2, -0.5);                        // -1/2!
+    c = __builtin_fma(c, r2, 1.0);
+    const int q = int(int64_t(k) & 3);
+    return q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+}
+
+// FP64 variant: the orbital offset from the FP64 sine, rounded to the nearest sample.
 __device__ __forceinline__ int source_index_fp64(uint32_t i, double tau_dt, double omega, double psi,
                                                  double dt, double s0_dt) {
     const double t = double(i) * dt;
-    const double off = tau_dt * sin(omega * t + psi) - s0_dt;
+    const double off = tau_dt * sin_f64(omega * t + psi) - s0_dt;
     return int(floor(double(i) - off + 0.5));
 }
 

@@ -75,6 +75,8 @@ bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* shape);
 void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a);
 void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a);
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles);
+// bytes / 16 uint4 copies from src to dst (a multiple of 16 bytes).
+void copy16(hipStream_t st, const void* src, void* dst, size_t bytes);
 // Copies {count, flags} and the first `keep` candidates of each of `nslots` slots into a
 // dense staging area of (keep + 1) x 16 bytes per slot.
 void pack_results(hipStream_t st, const uint32_t* results, uint32_t slot_words, uint32_t nslots, uint32_t keep,

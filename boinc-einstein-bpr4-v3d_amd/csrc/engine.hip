@@ -567,6 +567,43 @@ int brp_hip_kernel_ms(brp_hip* h, double out[4]) {
     });
 }
 
+int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps) {
+    return guarded([&] {
+        require(gbps && bytes >= 16 && reps > 0, "invalid copy calibration");
+        bytes &= ~size_t(15);
+        void *a = nullptr, *b = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = [&] {
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+            if (st) hipStreamDestroy(st);
+            if (a) hipFree(a);
+            if (b) hipFree(b);
+        };
+        try {
+            HIPC(hipMalloc(&a, bytes));
+            HIPC(hipMalloc(&b, bytes));
+            HIPC(hipMemset(a, 1, bytes));
+            HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            HIPC(hipEventCreate(&e0));
+            HIPC(hipEventCreate(&e1));
+            brp_engine::copy16(st, a, b, bytes);  // warm-up
+            HIPC(hipEventRecord(e0, st));
+            for (int r = 0; r < reps; ++r) brp_engine::copy16(st, a, b, bytes);
+            HIPC(hipEventRecord(e1, st));
+            HIPC(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPC(hipEventElapsedTime(&ms, e0, e1));
+            *gbps = 2.0 * double(bytes) * reps / (double(ms) * 1e-3) / 1e9;
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
+    });
+}
+
 int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint32_t samples, double sample_us,
                         double f0, double false_alarm, uint32_t window, brp_toplist* top, brp_params* templates_out,
                         uint32_t* cand_count, double* gpu_ms) {

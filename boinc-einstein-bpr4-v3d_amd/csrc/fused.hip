@@ -76,9 +76,14 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t grid) {
     return (bid % 8u) * (grid / 8u) + bid / 8u;
 }
 
-// LDS slot of element l of sequence s (sequences interleaved, element-major).
+// LDS slot of element l of sequence s: sequences interleaved, element-major, with one
+// padding row per 16 elements.  The first Stockham stage stores element 16 j + q from
+// lane j, a stride of 16 S elements; the padding turns that into 17 S, so the lanes of
+// a wave fall into different LDS banks.
 template <int S>
-__device__ __forceinline__ int lds_at(int l, int s) { return l * S + s; }
+__device__ __forceinline__ int lds_at(int l, int s) { return (l + (l >> 4)) * S + s; }
+template <int L, int S>
+constexpr int lds_size() { return (L + L / 16) * S; }
 
 // Stockham stage ST of an L-point FFT over S sequences held by T threads, E = L S / T
 // elements each.  On entry v[u R + q] holds input q of butterfly u = (s, j), b = t + T u,
@@ -131,13 +136,19 @@ __device__ __forceinline__ cf twiddle2(const float2* __restrict__ lo, const floa
 }
 
 // ------------------------------------------------------------------ pass 1 --
-template <int C, int S, int T>
+// SNAP: also write the resampled samples and their source indices (validation runs).
+// F64: orbital offset from the FP64 sine (a reported variant; the FP32 LUT rule is the
+// parity mode).  The hot instance has neither, and its gather is branch-free: every
+// index is computed first, loads use the index clamped into the observation and the
+// value is masked afterwards, so all of a thread's loads can be in flight together.
+template <int C, int S, int T, bool SNAP, bool F64>
 __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
     constexpr int E = C * S / T;
     static_assert(plan_ok<C>(E), "pass-1 radix plan does not fit the block");
     constexpr Radices pl = radices_for(C);
     constexpr int R0 = pl.r[0], RL = pl.r[pl.count - 1];
-    __shared__ float2 lds[C * S];
+    __shared__ float2 lds[lds_size<C, S>()];
+    __shared__ float2 lut[65];
     __shared__ double red[T / 64];
     const int t = threadIdx.x;
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
@@ -148,24 +159,39 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
     const uint32_t r0 = rb * S;
     const uint32_t steps = p.steps, U = p.unpadded;
     const Orbit32 o{p.tau, p.omega, p.psi, p.dt, p.step_inv, p.s0, p.flags};
-    const double tau_dt = double(p.tau) / double(p.dt), s0_dt = double(p.tau) * sin(double(p.psi)) / double(p.dt);
-    float* snap_y = a.snap_resampled ? a.snap_resampled + size_t(slot) * sh.n : nullptr;
-    int32_t* snap_j = a.snap_index ? a.snap_index + size_t(slot) * sh.n : nullptr;
+    if (t < 65) lut[t] = a.lut[t];
+    __syncthreads();
+
+    // Indices of the thread's 2E samples, in register order.
+    int src[2 * E];
+#pragma unroll
+    for (int u = 0; u < E / R0; ++u) {
+        const int b = t + T * u;
+        const int s = b % S, j = b / S;
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+            const uint32_t m = r0 + s + R * uint32_t(j + q * (C / R0));
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t i = 2 * m + h;
+                int jj;
+                if constexpr (F64) {
+                    const double tau_dt = double(p.tau) / double(p.dt);
+                    const double s0_dt = double(p.tau) * sin(double(p.psi)) / double(p.dt);
+                    jj = source_index_fp64(i, tau_dt, double(p.omega), double(p.psi), double(p.dt), s0_dt);
+                } else {
+                    jj = source_index(i, o, lut);
+                }
+                src[2 * (u * R0 + q) + h] = jj;
+            }
+        }
+    }
+    float y[2 * E];
+#pragma unroll
+    for (int e = 0; e < 2 * E; ++e) y[e] = a.x[min(max(src[e], 0), int(U) - 1)];
 
     double acc = 0.0;
     bool bad = false;
-    auto sample = [&](uint32_t i) -> float {
-        if (i >= steps) return 0.0f;
-        const int j = a.fp64_phase ? source_index_fp64(i, tau_dt, double(p.omega), double(p.psi), double(p.dt), s0_dt)
-                                   : source_index(i, o, a.lut);
-        float y = 0.0f;
-        if (j < 0 || uint32_t(j) >= U) bad = true;
-        else y = a.x[j];
-        acc += double(y);
-        if (snap_y) { snap_y[i] = y; snap_j[i] = j; }
-        return y;
-    };
-
     cf v[E];
 #pragma unroll
     for (int u = 0; u < E / R0; ++u) {
@@ -173,11 +199,25 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
         const int s = b % S, j = b / S;
 #pragma unroll
         for (int q = 0; q < R0; ++q) {
-            const uint32_t c = uint32_t(j + q * (C / R0));
-            const uint32_t m = r0 + s + R * c;
-            const float y0 = sample(2 * m);
-            const float y1 = sample(2 * m + 1);
-            v[u * R0 + q] = cf{y0, y1};
+            const uint32_t m = r0 + s + R * uint32_t(j + q * (C / R0));
+            float pair[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t i = 2 * m + h;
+                const int e = 2 * (u * R0 + q) + h;
+                const bool in_series = i < steps;
+                const bool inside = src[e] >= 0 && uint32_t(src[e]) < U;
+                bad |= in_series && !inside;
+                pair[h] = (in_series && inside) ? y[e] : 0.0f;
+                acc += double(pair[h]);
+                if constexpr (SNAP) {
+                    if (in_series) {
+                        a.snap_resampled[size_t(slot) * sh.n + i] = pair[h];
+                        a.snap_index[size_t(slot) * sh.n + i] = src[e];
+                    }
+                }
+            }
+            v[u * R0 + q] = cf{pair[0], pair[1]};
         }
     }
     fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
@@ -206,25 +246,17 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
     }
 }
 
-// cot(theta/2) for W = e^{-i theta} = (wr, wi), 0 < theta < 2 pi, in the better
-// conditioned of its two half-angle forms.
-__device__ __forceinline__ float half_cot(float wr, float wi) {
-    const bool front = wr >= 0.0f;
-    const float num = front ? 1.0f + wr : -wi;
-    const float den = front ? -wi : 1.0f - wr;
-    return __fdividef(num, den);
-}
-
 // ------------------------------------------------------------------ pass 2 --
-template <int R, int S, int T>
+// SNAP: also write the spectrum X (validation runs).
+template <int R, int S, int T, bool SNAP>
 __global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
     constexpr int SS = 2 * S;
     constexpr int E = R * SS / T;
     static_assert(plan_ok<R>(E), "pass-2 radix plan does not fit the block");
     constexpr Radices pl = radices_for(R);
     constexpr int R0 = pl.r[0], RL = pl.r[pl.count - 1];
-    __shared__ float2 lds[R * SS];
-    __shared__ double red[T];
+    __shared__ float2 lds[lds_size<R, SS>()];
+    __shared__ double red[T / 64 + 1];
     const int t = threadIdx.x;
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t g = item / sh.blocks2, w = item % sh.blocks2;
@@ -234,16 +266,19 @@ __global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
     const bool middle = w == C / (2 * S);  // the block that owns the self-paired column C/2
     const uint32_t base = middle ? C / 2 : w * S;
 
-    // Pad mean from the pass-1 partial sums, fixed order.
+    // Pad mean from the pass-1 partial sums, in a fixed order (bit-reproducible).
     double part = 0.0;
     for (uint32_t b = t; b < sh.blocks1; b += T) part += a.partial[size_t(g) * sh.blocks1 + b];
-    red[t] = part;
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+    if ((t & 63) == 0) red[t >> 6] = part;
     __syncthreads();
-    for (int d = T / 2; d > 0; d >>= 1) {
-        if (t < d) red[t] += red[t + d];
-        __syncthreads();
+    if (t == 0) {
+        double total = 0.0;
+        for (int wv = 0; wv < T / 64; ++wv) total += red[wv];
+        red[T / 64] = total;
     }
-    const float mu = float(red[0] / double(p.steps));
+    __syncthreads();
+    const float mu = float(red[T / 64] / double(p.steps));
     if (w == 0 && t == 0) a.mean[slot] = mu;
 
     auto column = [&](int s) -> uint32_t {
@@ -273,13 +308,16 @@ __global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
     __syncthreads();
 
     float* P = a.P + size_t(g) * M;
-    float2* snapX = a.snap_X ? a.snap_X + size_t(slot) * (M + 1) : nullptr;
+    float2* snapX = SNAP ? a.snap_X + size_t(slot) * (M + 1) : nullptr;
     const float inv_n = 1.0f / float(N);
     const uint32_t steps = p.steps;
     const float pad_len = float(N - steps);
     const float parity = (steps & 1u) ? -1.0f : 1.0f;  // W_N^{M steps} = (-1)^steps
-    const double inv_nd = 1.0 / double(N);
-    for (int pi = t; pi < R * S; pi += T) {
+    const bool n_pow2 = (N & (N - 1)) == 0;
+    const uint32_t lg = __builtin_ctz(N);  // N = 2^lg or 3 * 2^lg
+#pragma unroll
+    for (int u = 0; u < R * S / T; ++u) {
+        const int pi = t + T * u;
         const int s = pi % S;
         const uint32_t k1 = uint32_t(pi / S);
         if (middle && s != 0) continue;
@@ -297,22 +335,33 @@ __global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
             xk.x += mu * pad_len;
             xm.x += mu * (parity - 1.0f) * 0.5f;  // U[M] = ((-1)^steps - 1) / 2
         } else {
-            // e = k steps mod N, exact: the product is below 2^53.
-            const uint64_t prod = uint64_t(k) * steps;
-            int64_t e = int64_t(prod) - int64_t(floor(double(prod) * inv_nd)) * int64_t(N);
-            if (e < 0) e += N;
-            if (e >= int64_t(N)) e -= N;
-            const cf we = twiddle2(a.twn_lo, a.twn_hi, uint32_t(e));
-            const float ck = half_cot(W.x, W.y), cm = half_cot(Wm.x, Wm.y);
+            // e = k steps mod N.  N = 2^lg: the 32-bit product wraps modulo a multiple of N.
+            // N = 3 * 2^lg: x = hi 2^lg + lo with hi < 2^25, and x mod N = (hi mod 3) 2^lg + lo.
+            uint32_t e;
+            if (n_pow2) {
+                e = (k * steps) & (N - 1);
+            } else {
+                const uint64_t x = uint64_t(k) * steps;
+                const uint32_t hi = uint32_t(x >> lg);
+                const uint32_t third = __umulhi(hi, 0xAAAAAAABu) >> 1;
+                e = ((hi - 3u * third) << lg) | (uint32_t(x) & ((1u << lg) - 1u));
+            }
+            const cf we = twiddle2(a.twn_lo, a.twn_hi, e);
+            // cot(pi k / N) and tan(pi k / N) = cot(pi (M - k) / N) from W_N^k = (cos, -sin),
+            // each in the half-angle form without cancellation for its half of (0, pi).
+            const bool front = W.x >= 0.0f;
+            const float num = front ? 1.0f + W.x : -W.y;
+            const float den = front ? -W.y : 1.0f - W.x;
+            const float ck = num * __builtin_amdgcn_rcpf(den);
+            const float cm = den * __builtin_amdgcn_rcpf(num);
             const cf uk = mul(cf{we.x - 1.0f, we.y}, cf{0.5f, -0.5f * ck});
-            const cf wem = cf{parity * we.x, -parity * we.y};
-            const cf um = mul(cf{wem.x - 1.0f, wem.y}, cf{0.5f, -0.5f * cm});
+            const cf um = mul(cf{parity * we.x - 1.0f, -parity * we.y}, cf{0.5f, -0.5f * cm});
             xk = cf{__builtin_fmaf(mu, uk.x, xk.x), __builtin_fmaf(mu, uk.y, xk.y)};
             xm = cf{__builtin_fmaf(mu, um.x, xm.x), __builtin_fmaf(mu, um.y, xm.y)};
         }
         P[k] = k == 0 ? 0.0f : (xk.x * xk.x + xk.y * xk.y) * inv_n;
         if (!self) P[M - k] = (xm.x * xm.x + xm.y * xm.y) * inv_n;
-        if (snapX) {
+        if constexpr (SNAP) {
             store(snapX + k, xk);
             if (!self || k == 0) store(snapX + (M - k), xm);
         }
@@ -320,7 +369,20 @@ __global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
 }
 
 // ---------------------------------------------------------------- harmonic --
-template <int T>
+// Bins i < H < 2^24, so every index of the dyadic sums, (i k + 8) >> 4, is one 24-bit
+// multiply-add and a shift, and the loads use 32-bit offsets from the template's power.
+__device__ __forceinline__ void dyadic_sums24(const float* __restrict__ P, uint32_t i, float s[4]) {
+#define BRP_AT(k) P[(__umul24(i, (k)) + 8u) >> 4]
+    float v = P[i];
+    v += BRP_AT(8u); s[0] = v;
+    v += BRP_AT(12u) + BRP_AT(4u); s[1] = v;
+    v += BRP_AT(14u) + BRP_AT(10u) + BRP_AT(6u) + BRP_AT(2u); s[2] = v;
+    v += BRP_AT(15u) + BRP_AT(13u) + BRP_AT(11u) + BRP_AT(9u) + BRP_AT(7u) + BRP_AT(5u) + BRP_AT(3u) + BRP_AT(1u);
+    s[3] = v;
+#undef BRP_AT
+}
+
+template <int T, bool SNAP>
 __global__ __launch_bounds__(T) void k_harmonic(FusedArgs a, FusedShape sh, uint32_t tiles) {
     __shared__ float sums[4][kTile + 8];
     const int t = threadIdx.x;
@@ -328,54 +390,51 @@ __global__ __launch_bounds__(T) void k_harmonic(FusedArgs a, FusedShape sh, uint
     const uint32_t g = item / tiles, tile = item % tiles;
     const uint32_t slot = a.first + g;
     const brp_params& p = a.params[slot];
-    const float* P = a.P + size_t(g) * sh.m;
-    const int64_t i0 = int64_t(tile) * kTile;
-    const uint32_t W = p.window, H = p.harmonic, F = p.fundamental;
+    const float* __restrict__ P = a.P + size_t(g) * sh.m;
+    const int i0 = int(tile) * kTile;
+    const int W = int(p.window), H = int(p.harmonic), F = int(p.fundamental);
+    const float thr0 = p.threshold[0];
     uint32_t* res = a.results + size_t(slot) * a.slot_words;
-    float* snap = a.snap_summed ? a.snap_summed + size_t(slot) * 5 * a.snap_f : nullptr;
+    float* snap = SNAP ? a.snap_summed + size_t(slot) * 5 * a.snap_f : nullptr;
 
+    // Phase A: dyadic sums of bins i0 - 8 .. i0 + kTile - 1 into LDS (-inf outside
+    // [W, H)); the level-0 test of the fundamentals j = i in [i0, i0 + kTile) rides along.
     for (int idx = t; idx < kTile + 8; idx += T) {
-        const int64_t i = i0 - 8 + idx;
-        if (i >= int64_t(W) && i < int64_t(H)) {
-            float s[5];
-            dyadic_sums(P, uint32_t(i), s);
-            sums[0][idx] = s[1]; sums[1][idx] = s[2]; sums[2][idx] = s[3]; sums[3][idx] = s[4];
-        } else {
-            sums[0][idx] = -INFINITY; sums[1][idx] = -INFINITY; sums[2][idx] = -INFINITY; sums[3][idx] = -INFINITY;
-        }
-    }
-    // Level 0: the power itself.
-    for (int jj = t; jj < kTile; jj += T) {
-        const uint32_t j = uint32_t(i0) + jj;
-        if (j < F) {
-            const float v = P[j];
-            if (snap) snap[j] = v;
-            if (j >= W && v > p.threshold[0]) append_candidate(res, p.capacity, j, 0, v);
+        const int i = i0 - 8 + idx;
+        float s[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        if (i >= W && i < H) dyadic_sums24(P, uint32_t(i), s);
+        sums[0][idx] = s[0]; sums[1][idx] = s[1]; sums[2][idx] = s[2]; sums[3][idx] = s[3];
+        if (idx >= 8 && i < F) {
+            const float v = P[i];
+            if constexpr (SNAP) snap[i] = v;
+            if (i >= W && v > thr0) append_candidate(res, p.capacity, uint32_t(i), 0, v);
         }
     }
     __syncthreads();
+    // Phase B: level h, fundamental j owns bins [j 2^h - 2^(h-1), j 2^h + 2^(h-1)) ∩ [W, H);
+    // the tile owns the j with j 2^h in [i0, i0 + kTile), whose bins lie in its LDS rows.
 #pragma unroll
-    for (uint32_t h = 1; h <= 4; ++h) {
-        const int64_t width = int64_t(1) << h;
-        const uint32_t groups = kTile >> h;
-        const int64_t j0 = i0 >> h;
-        for (uint32_t gi = t; gi < groups; gi += T) {
-            const int64_t j = j0 + gi;
-            if (j >= int64_t(F)) continue;
-            const int64_t begin = max(int64_t(W), j * width - width / 2);
-            const int64_t end = min(int64_t(H), j * width + width / 2);
+    for (int h = 1; h <= 4; ++h) {
+        const int width = 1 << h;
+        const int groups = kTile >> h;
+        const int j0 = i0 >> h;
+        const float thr = p.threshold[h];
+        for (int gi = t; gi < groups; gi += T) {
+            const int j = j0 + gi;
+            if (j >= F) continue;
+            const int begin = max(W, j * width - width / 2);
+            const int end = min(H, j * width + width / 2);
             float value = 0.0f;
             if (begin < end) {
-                // begin >= i0 - 8 and end <= i0 + kTile: inside the tile's sums.
                 const float* row = sums[h - 1];
-                const int lo = int(begin - i0 + 8), hi = int(end - i0 + 8);
+                const int lo = begin - i0 + 8, hi = end - i0 + 8;
                 const float first = row[lo];
                 float best = first;
                 for (int i = lo + 1; i < hi; ++i) best = fmaxf(best, row[i]);
-                value = best > p.threshold[h] ? best : first;
+                value = best > thr ? best : first;
             }
-            if (snap) snap[size_t(h) * a.snap_f + j] = value;
-            if (j >= int64_t(W) && value > p.threshold[h]) append_candidate(res, p.capacity, uint32_t(j), h, value);
+            if constexpr (SNAP) snap[size_t(h) * a.snap_f + j] = value;
+            if (j >= W && value > thr) append_candidate(res, p.capacity, uint32_t(j), uint32_t(h), value);
         }
     }
 }
@@ -392,15 +451,29 @@ __global__ void k_pack_results(const uint32_t* __restrict__ results, uint32_t sl
     for (uint32_t e = threadIdx.x; e <= n; e += blockDim.x) dst[e] = src[e];
 }
 
-template <int C, int S>
-void launch1(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
-    k_pass1<C, S, kThreads><<<sh.blocks1 * a.count, kThreads, 0, st>>>(a, sh);
-}
-template <int R, int S>
-void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
-    k_pass2<R, S, kThreads><<<sh.blocks2 * a.count, kThreads, 0, st>>>(a, sh);
+// Plain streaming copy, 16 bytes per lane per iteration (the counter calibration kernel).
+__global__ __launch_bounds__(256) void brp_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
 }
 
+template <int C, int S, int T>
+void launch1(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
+    const dim3 grid(sh.blocks1 * a.count), block(T);
+    const bool snap = a.snap_resampled != nullptr;
+    if (snap && a.fp64_phase) k_pass1<C, S, T, true, true><<<grid, block, 0, st>>>(a, sh);
+    else if (snap) k_pass1<C, S, T, true, false><<<grid, block, 0, st>>>(a, sh);
+    else if (a.fp64_phase) k_pass1<C, S, T, false, true><<<grid, block, 0, st>>>(a, sh);
+    else k_pass1<C, S, T, false, false><<<grid, block, 0, st>>>(a, sh);
+}
+template <int R, int S, int T>
+void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
+    if (a.snap_X) k_pass2<R, S, T, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
+    else k_pass2<R, S, T, false><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
+}
+
+// Per series length M = N/2: pass-1 length C with S1 rows per block, pass-2 length R
+// with S2 column pairs per block.  Blocks hold 16 elements per thread where the radix
+// plan allows (up to 1024 threads, 16 waves per CU), 48 for the 3 * 2^k plans.
 struct ShapeRow { uint32_t m, c, r, s1, s2; };
 constexpr ShapeRow kShapes[] = {
     {1u << 12, 64, 64, 64, 32},   {1u << 13, 128, 64, 32, 32},  {1u << 14, 128, 128, 32, 16},
@@ -433,35 +506,40 @@ bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* out) {
 
 void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
     switch (s.c * 1000 + s.s1) {
-        case 64 * 1000 + 64: launch1<64, 64>(st, s, a); break;
-        case 128 * 1000 + 32: launch1<128, 32>(st, s, a); break;
-        case 256 * 1000 + 16: launch1<256, 16>(st, s, a); break;
-        case 512 * 1000 + 16: launch1<512, 16>(st, s, a); break;
-        case 1024 * 1000 + 16: launch1<1024, 16>(st, s, a); break;
-        case 2048 * 1000 + 8: launch1<2048, 8>(st, s, a); break;
-        case 4096 * 1000 + 4: launch1<4096, 4>(st, s, a); break;
-        case 384 * 1000 + 32: launch1<384, 32>(st, s, a); break;
-        case 768 * 1000 + 16: launch1<768, 16>(st, s, a); break;
-        case 1536 * 1000 + 8: launch1<1536, 8>(st, s, a); break;
-        case 3072 * 1000 + 4: launch1<3072, 4>(st, s, a); break;
+        case 64 * 1000 + 64: launch1<64, 64, 256>(st, s, a); break;
+        case 128 * 1000 + 32: launch1<128, 32, 256>(st, s, a); break;
+        case 256 * 1000 + 16: launch1<256, 16, 256>(st, s, a); break;
+        case 512 * 1000 + 16: launch1<512, 16, 512>(st, s, a); break;
+        case 1024 * 1000 + 16: launch1<1024, 16, 1024>(st, s, a); break;
+        case 2048 * 1000 + 8: launch1<2048, 8, 1024>(st, s, a); break;
+        case 4096 * 1000 + 4: launch1<4096, 4, 1024>(st, s, a); break;
+        case 384 * 1000 + 32: launch1<384, 32, 256>(st, s, a); break;
+        case 768 * 1000 + 16: launch1<768, 16, 256>(st, s, a); break;
+        case 1536 * 1000 + 8: launch1<1536, 8, 512>(st, s, a); break;
+        case 3072 * 1000 + 4: launch1<3072, 4, 256>(st, s, a); break;
         default: break;
     }
 }
 
 void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
     switch (s.r * 1000 + s.s2) {
-        case 64 * 1000 + 32: launch2<64, 32>(st, s, a); break;
-        case 128 * 1000 + 16: launch2<128, 16>(st, s, a); break;
-        case 256 * 1000 + 16: launch2<256, 16>(st, s, a); break;
-        case 512 * 1000 + 16: launch2<512, 16>(st, s, a); break;
-        case 1024 * 1000 + 8: launch2<1024, 8>(st, s, a); break;
-        case 2048 * 1000 + 4: launch2<2048, 4>(st, s, a); break;
+        case 64 * 1000 + 32: launch2<64, 32, 256>(st, s, a); break;
+        case 128 * 1000 + 16: launch2<128, 16, 256>(st, s, a); break;
+        case 256 * 1000 + 16: launch2<256, 16, 512>(st, s, a); break;
+        case 512 * 1000 + 16: launch2<512, 16, 1024>(st, s, a); break;
+        case 1024 * 1000 + 8: launch2<1024, 8, 1024>(st, s, a); break;
+        case 2048 * 1000 + 4: launch2<2048, 4, 1024>(st, s, a); break;
         default: break;
     }
 }
 
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles) {
-    k_harmonic<kThreads><<<tiles * a.count, kThreads, 0, st>>>(a, s, tiles);
+    if (a.snap_summed) k_harmonic<kThreads, true><<<tiles * a.count, kThreads, 0, st>>>(a, s, tiles);
+    else k_harmonic<kThreads, false><<<tiles * a.count, kThreads, 0, st>>>(a, s, tiles);
+}
+
+void copy16(hipStream_t st, const void* src, void* dst, size_t bytes) {
+    brp_copy<<<2048, 256, 0, st>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), bytes / 16);
 }
 
 void pack_results(hipStream_t st, const uint32_t* results, uint32_t slot_words, uint32_t nslots, uint32_t keep,

@@ -70,8 +70,12 @@ double upper_tail(double x, uint32_t k) {
 }
 
 // Quantile of the upper tail: the x with Q(x; 2k) = q, by bisection on the monotone
-// log tail (the role gsl_cdf_chisq_Qinv plays at src/workload.cpp:122).
+// log tail (the role gsl_cdf_chisq_Qinv plays at src/workload.cpp:122).  Every template
+// of one series length asks for the same five quantiles, so the last answer per degree
+// count is kept.
 double upper_tail_quantile(double q, uint32_t k) {
+    thread_local double memo_q[17] = {}, memo_x[17] = {};
+    if (k <= 16 && memo_q[k] == q) return memo_x[k];
     const double goal = std::log(q);
     double lo = 0, hi = 1;
     while (log_upper_tail(hi, k) > goal) hi *= 2;
@@ -80,7 +84,9 @@ double upper_tail_quantile(double q, uint32_t k) {
         if (mid <= lo || mid >= hi) break;
         (log_upper_tail(mid, k) > goal ? lo : hi) = mid;
     }
-    return 0.5 * (lo + hi);
+    const double x = 0.5 * (lo + hi);
+    if (k <= 16) { memo_q[k] = q; memo_x[k] = x; }
+    return x;
 }
 
 uint64_t mix64(uint64_t z) {
@@ -106,10 +112,32 @@ float brp_sample_offset(uint32_t i, const brp_params* p) {
     return p->tau * lut_sin(p->omega * t + p->psi) * p->step_inv - p->s0;
 }
 
+// The largest steps <= unpadded - 1 with float(steps) - offset(steps) < unpadded - 1 in
+// the FP32 rule (src/science.cpp:41-45).  The reference scans down from unpadded - 1,
+// up to ~2 tau/dt iterations.  Here the scan starts 64 samples above the FP64 solution
+// s* of s - offset(s) = unpadded - 1: s - offset(s) grows with slope 1 - tau omega cos >
+// 0.5, and the FP32 rule differs from the FP64 model by well under one sample, so no
+// s beyond that start can satisfy the rule and the result is the reference's.
 int brp_resampling_steps(brp_params* p) {
     if (!p || p->unpadded < 2) return brp::fail(BRP_E_INVALID, "invalid template");
     const float last = float(p->unpadded - 1);
     uint32_t steps = p->unpadded - 1;
+    if (double(p->tau) * double(p->omega) < 0.5) {
+        auto offset = [&](double s) {
+            return double(p->tau) * std::sin(double(p->omega) * (s * double(p->dt)) + double(p->psi)) *
+                       double(p->step_inv) - double(p->s0);
+        };
+        // Fixed point s = (unpadded - 1) + offset(s): a contraction with factor tau omega < 1/2.
+        double s = double(p->unpadded - 1);
+        for (int it = 0; it < 200; ++it) {
+            const double next = double(p->unpadded - 1) + offset(s);
+            const bool done = std::fabs(next - s) < 1e-3;
+            s = next;
+            if (done) break;
+        }
+        const double start = std::floor(s) + 64.0;
+        if (start < double(steps)) steps = start < 1.0 ? 1u : uint32_t(start);
+    }
     while (steps && float(steps) - brp_sample_offset(steps, p) >= last) --steps;
     p->steps = steps;
     if (!steps) return brp::fail(BRP_E_INVALID, "template produces empty series");

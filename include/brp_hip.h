@@ -173,6 +173,10 @@ size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap);
 /* Per-kernel device times of the last run (ms), summed over the batch, as measured
  * with HIP events on the launch stream: pass1/resample, fft, power, harmonic. */
 int brp_hip_kernel_ms(brp_hip* h, double out[4]);
+/* Measurement aid with no reference counterpart: copies `bytes` between two device
+ * buffers `reps` times with a plain streaming kernel (brp_copy) and returns the mean
+ * GB/s; used to calibrate hardware byte counters against a transfer of known size. */
+int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps);
 
 /* Bank search: the brp_workunit bank loop (src/workunit_main.cpp:77-136) for a bank of
  * `count` orbits against the handle's series, in batches of options.batch templates,

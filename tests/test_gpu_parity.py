@@ -167,6 +167,33 @@ def test_bank_toplist_identical(brp, oracle, wl):
     assert sum(r.harmonics > 0 for r in ref.state) > 0
 
 
+def test_fp64_phase_variant(brp, oracle, wl):
+    """The FP64-phase variant rounds the exact orbit model; its index differences from the
+    FP32 LUT rule (the parity mode) are counted and reported, and its spectrum is checked
+    against the FP64 FFT of its own resampled series."""
+    w = wl.bank(count=2, n=1 << 20)
+    p = w.templates[0]
+    say("[gpu] fused FP64-phase variant, N=2^20")
+    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True, fp64_phase=True) as g:
+        g.set_template(p)
+        g.run()
+        snap = g.snapshot(0)
+    steps = p.steps
+    i = np.arange(steps, dtype=np.float64)
+    tau_dt = float(np.float32(p.tau)) / float(np.float32(p.dt))
+    s0_dt = float(np.float32(p.tau)) * np.sin(float(np.float32(p.psi))) / float(np.float32(p.dt))
+    off = tau_dt * np.sin(float(np.float32(p.omega)) * (i * float(np.float32(p.dt))) + float(np.float32(p.psi))) - s0_dt
+    resid = i - off - snap.index
+    assert np.max(np.abs(resid)) <= 0.5 + 1e-6
+    _, idx32 = oracle.resample(w.x, p)
+    differ = int(np.count_nonzero(idx32 != snap.index))
+    say({"fp64_phase_index_differences": differ, "of": steps})
+    assert differ < 0.05 * steps
+    assert np.array_equal(snap.resampled[:steps], w.x[snap.index])
+    _, p64 = oracle.power(snap.resampled)
+    assert parity.power_error(snap.power, p64)["ok"]
+
+
 def test_candidate_overflow_raises(brp, wl):
     w = wl.plain_small(threshold_scale=0.2)
     p = w.templates[0]

@@ -48,6 +48,17 @@ def test_parameters_equal_oracle(brp, oracle, n):
         assert _bytes(a) == _bytes(b)
 
 
+def test_resampling_steps_equal_oracle_on_extreme_orbits(brp, oracle):
+    # Short periods and large tau (tau * omega up to ~0.19) stress the jump-started scan.
+    rng = np.random.default_rng(99)
+    for n in (1 << 14, 3 << 13, 1 << 20):
+        for _ in range(40):
+            tau, period, psi = rng.uniform(0, 0.3), rng.uniform(10, 200), rng.uniform(0, 6.28)
+            a = brp.parameters(n, n - int(rng.integers(0, 1000)), tau, period, psi, 64e-6)
+            b = oracle.parameters(n, a.unpadded, tau, period, psi, 64e-6)
+            assert _bytes(a) == _bytes(b)
+
+
 def test_parameter_validation_messages(brp):
     with pytest.raises(RuntimeError, match="invalid observation/template"):
         brp.parameters(100, 100, 0.01, 100.0, 0.1, 64e-6)
