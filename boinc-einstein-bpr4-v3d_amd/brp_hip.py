@@ -117,7 +117,7 @@ def load() -> ctypes.CDLL:
         "brp_hip_device_info": (sz, [ctypes.c_void_p, ctypes.c_char_p, sz]),
         "brp_hip_allocated_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
         "brp_hip_profile_counters": (sz, [ctypes.c_void_p, ctypes.c_char_p, sz]),
-        "brp_hip_kernel_ms": (i32, [ctypes.c_void_p, P(ctypes.c_double)]),
+        "brp_hip_kernel_ms": (i32, [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_uint64)]),
         "brp_hip_copy_bandwidth": (i32, [sz, i32, P(ctypes.c_double)]),
         "brp_hip_bank_search": (i32, [ctypes.c_void_p, P(Orbit), sz, u32, f64, f64, f64, u32, ctypes.c_void_p,
                                       P(Params), P(u32), P(ctypes.c_double)]),
@@ -351,10 +351,13 @@ class GPU:
         load().brp_hip_profile_counters(self._h, buf, 1024)
         return buf.value.decode()
 
-    def kernel_ms(self) -> list[float]:
+    def kernel_ms(self) -> tuple[list[float], int]:
+        """Per-stage device ms since the previous call (which resets them) and the number
+        of launch sequences (groups or templates) they cover."""
         out = (ctypes.c_double * 4)()
-        _check(load().brp_hip_kernel_ms(self._h, out))
-        return list(out)
+        n = ctypes.c_uint64()
+        _check(load().brp_hip_kernel_ms(self._h, out, ctypes.byref(n)))
+        return list(out), n.value
 
     def bank_search(self, orbits, samples: int, sample_us: float, top: Toplist, f0: float = 400,
                     false_alarm: float = 0.08, window: int = 1000):

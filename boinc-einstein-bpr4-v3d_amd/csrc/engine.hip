@@ -90,7 +90,9 @@ struct brp_hip {
     hipStream_t stream = nullptr;
     std::vector<void*> allocations;
     uint64_t allocated = 0;
-    std::vector<brp_params> templates;
+    // Two halves of slots, parameters, staging and graph: the public calls use half 0; the
+    // bank loop alternates halves so that one batch runs while the previous is retained.
+    std::vector<brp_params> templates[2];
 
     float* x = nullptr;
     float2* lut = nullptr;
@@ -111,15 +113,20 @@ struct brp_hip {
     brp_engine::FusedArgs fa;
     uint32_t group = 1;
     uint32_t tiles = 0;
-    hipGraphExec_t graph_exec = nullptr;
-    uint32_t graph_count = 0;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    uint32_t graph_count[2] = {0, 0};
     bool graph_events = true;
 
-    std::vector<hipEvent_t> events;
-    std::vector<double> stage_ms_per_template;  // 4 per template of the last run
-    double kernel_ms[4] = {0, 0, 0, 0};
-    bool ran = false;
-    std::vector<std::vector<brp_candidate>> extra;  // full lists fetched beyond `keep`
+    std::vector<hipEvent_t> events[2];
+    hipEvent_t done[2] = {nullptr, nullptr};
+    std::vector<double> stage_ms_per_template[2];  // 4 per template of the last run
+    double kernel_ms[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    bool ran[2] = {false, false};
+    double submit_ms = 0;
+    double total_ms[4] = {0, 0, 0, 0};  // since the last brp_hip_kernel_ms
+    uint64_t total_launches = 0;
+
+    size_t slot_base(int half) const { return size_t(half) * opt.batch; }
 
     template <class T>
     T* alloc(size_t count) {
@@ -136,18 +143,19 @@ struct brp_hip {
         HIPC(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
         return p;
     }
-    hipEvent_t event(size_t i) {
-        while (events.size() <= i) {
+    hipEvent_t event(int half, size_t i) {
+        while (events[half].size() <= i) {
             hipEvent_t e;
             HIPC(hipEventCreate(&e));
-            events.push_back(e);
+            events[half].push_back(e);
         }
-        return events[i];
+        return events[half][i];
     }
     ~brp_hip() {
         if (stream) hipStreamSynchronize(stream);
-        if (graph_exec) hipGraphExecDestroy(graph_exec);
-        for (hipEvent_t e : events) hipEventDestroy(e);
+        for (auto g : graph_exec) if (g) hipGraphExecDestroy(g);
+        for (auto& list : events) for (hipEvent_t e : list) hipEventDestroy(e);
+        for (hipEvent_t e : done) if (e) hipEventDestroy(e);
         for (void* p : allocations) hipFree(p);
         if (staging_host) hipHostFree(staging_host);
         if (params_host) hipHostFree(params_host);
@@ -224,7 +232,7 @@ void create_fused(brp_hip& h) {
     a.Y = h.alloc<float2>(size_t(h.group) * s.m);
     a.P = h.alloc<float>(size_t(h.group) * s.m);
     a.partial = h.alloc<double>(size_t(h.group) * s.blocks1);
-    a.mean = h.alloc<float>(h.opt.batch);
+    a.mean = h.alloc<float>(2 * size_t(h.opt.batch));
     a.results = h.results;
     a.slot_words = h.slot_words;
     a.params = h.params_dev;
@@ -245,37 +253,40 @@ void create_fused(brp_hip& h) {
 // The plain launch sequence for slot r (src/vulkan.cpp:221-266), with events at the
 // four stage boundaries.
 void enqueue_plain(brp_hip& h, uint32_t r) {
-    const brp_params& p = h.templates[r];
+    const brp_params& p = h.templates[0][r];
     hipStream_t st = h.stream;
     uint32_t* slot = h.results + size_t(r) * h.slot_words;
     brp_engine::plain_clear_slots(st, slot, h.slot_words, 1);
-    HIPC(hipEventRecord(h.event(5 * r), st));
+    HIPC(hipEventRecord(h.event(0, 5 * r), st));
     brp_engine::plain_resample(st, h.pb, p, slot);
-    HIPC(hipEventRecord(h.event(5 * r + 1), st));
+    HIPC(hipEventRecord(h.event(0, 5 * r + 1), st));
     const bool fused_power = h.opt.fused_power != 0;
     float2* z = brp_engine::plain_fft(st, h.pb, h.opt.radix8 != 0, h.opt.radix4 != 0 || h.opt.radix8 != 0);
     brp_engine::plain_spectrum(st, h.pb, z, fused_power, h.power_buf);
-    HIPC(hipEventRecord(h.event(5 * r + 2), st));
+    HIPC(hipEventRecord(h.event(0, 5 * r + 2), st));
     if (!fused_power || !h.pb.packed) brp_engine::plain_power(st, h.pb, h.power_buf);
-    HIPC(hipEventRecord(h.event(5 * r + 3), st));
+    HIPC(hipEventRecord(h.event(0, 5 * r + 3), st));
     brp_engine::plain_harmonic(st, h.power_buf, h.power_buf + p.n / 2, p, h.opt.scatter != 0, slot);
-    HIPC(hipEventRecord(h.event(5 * r + 4), st));
+    HIPC(hipEventRecord(h.event(0, 5 * r + 4), st));
 }
 
-// The fused sequence: every group of `group` templates in three launches.
-void enqueue_fused(brp_hip& h, uint32_t count, bool capturing) {
+// The fused sequence for `count` templates in slots of `half`: every group of `group`
+// templates in three launches, then the packing of the results for one readback.
+void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
     hipStream_t st = h.stream;
-    brp_engine::plain_clear_slots(st, h.results, h.slot_words, count);
+    const size_t base = h.slot_base(half);
+    uint32_t* results = h.results + base * h.slot_words;
+    brp_engine::plain_clear_slots(st, results, h.slot_words, count);
     const bool ev = !capturing || h.graph_events;
     auto record = [&](size_t i) {
         if (!ev) return;
-        if (capturing) HIPC(hipEventRecordWithFlags(h.event(i), st, hipEventRecordExternal));
-        else HIPC(hipEventRecord(h.event(i), st));
+        if (capturing) HIPC(hipEventRecordWithFlags(h.event(half, i), st, hipEventRecordExternal));
+        else HIPC(hipEventRecord(h.event(half, i), st));
     };
     uint32_t gi = 0;
     for (uint32_t first = 0; first < count; first += h.group, ++gi) {
         brp_engine::FusedArgs a = h.fa;
-        a.first = first;
+        a.first = uint32_t(base) + first;
         a.count = std::min(h.group, count - first);
         record(4 * gi);
         brp_engine::fused_pass1(st, h.shape, a);
@@ -285,22 +296,22 @@ void enqueue_fused(brp_hip& h, uint32_t count, bool capturing) {
         brp_engine::fused_harmonic(st, h.shape, a, h.tiles);
         record(4 * gi + 3);
     }
-    brp_engine::pack_results(st, h.results, h.slot_words, count, h.keep, h.staging_dev);
+    brp_engine::pack_results(st, results, h.slot_words, count, h.keep, h.staging_dev + base * (h.keep + 1) * 4);
 }
 
-void run_fused(brp_hip& h, uint32_t count) {
+void run_fused(brp_hip& h, int half, uint32_t count) {
     if (!h.opt.graph || h.opt.snapshot) {
-        enqueue_fused(h, count, false);
+        enqueue_fused(h, half, count, false);
         return;
     }
-    if (!h.graph_exec || h.graph_count != count) {
-        if (h.graph_exec) { HIPC(hipGraphExecDestroy(h.graph_exec)); h.graph_exec = nullptr; }
+    if (!h.graph_exec[half] || h.graph_count[half] != count) {
+        if (h.graph_exec[half]) { HIPC(hipGraphExecDestroy(h.graph_exec[half])); h.graph_exec[half] = nullptr; }
         hipGraph_t graph = nullptr;
         auto capture = [&](bool events) {
             h.graph_events = events;
             HIPC(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
             try {
-                enqueue_fused(h, count, true);
+                enqueue_fused(h, half, count, true);
             } catch (...) {
                 hipGraph_t dropped = nullptr;
                 hipStreamEndCapture(h.stream, &dropped);
@@ -315,23 +326,25 @@ void run_fused(brp_hip& h, uint32_t count) {
             (void)hipGetLastError();
             capture(false);  // this runtime cannot record event nodes: time the whole replay
         }
-        HIPC(hipGraphInstantiate(&h.graph_exec, graph, nullptr, nullptr, 0));
+        HIPC(hipGraphInstantiate(&h.graph_exec[half], graph, nullptr, nullptr, 0));
         HIPC(hipGraphDestroy(graph));
-        h.graph_count = count;
+        h.graph_count[half] = count;
     }
-    HIPC(hipGraphLaunch(h.graph_exec, h.stream));
+    HIPC(hipGraphLaunch(h.graph_exec[half], h.stream));
 }
 
-void fill_timings(brp_hip& h, uint32_t count) {
-    h.stage_ms_per_template.assign(4 * size_t(count), 0.0);
-    std::fill(h.kernel_ms, h.kernel_ms + 4, 0.0);
+void fill_timings(brp_hip& h, int half, uint32_t count) {
+    auto& per = h.stage_ms_per_template[half];
+    double* km = h.kernel_ms[half];
+    per.assign(4 * size_t(count), 0.0);
+    std::fill(km, km + 4, 0.0);
     float ms = 0;
     if (h.opt.engine == BRP_ENGINE_PLAIN) {
         for (uint32_t r = 0; r < count; ++r)
             for (uint32_t s = 0; s < 4; ++s) {
-                HIPC(hipEventElapsedTime(&ms, h.event(5 * r + s), h.event(5 * r + s + 1)));
-                h.stage_ms_per_template[4 * r + s] = ms;
-                h.kernel_ms[s] += ms;
+                HIPC(hipEventElapsedTime(&ms, h.event(half, 5 * r + s), h.event(half, 5 * r + s + 1)));
+                per[4 * r + s] = ms;
+                km[s] += ms;
             }
         return;
     }
@@ -340,31 +353,75 @@ void fill_timings(brp_hip& h, uint32_t count) {
     for (uint32_t gi = 0; gi < groups; ++gi) {
         const uint32_t first = gi * h.group, n = std::min(h.group, count - first);
         double stage[4] = {0, 0, 0, 0};
-        HIPC(hipEventElapsedTime(&ms, h.event(4 * gi), h.event(4 * gi + 1)));
+        HIPC(hipEventElapsedTime(&ms, h.event(half, 4 * gi), h.event(half, 4 * gi + 1)));
         stage[0] = ms;
-        HIPC(hipEventElapsedTime(&ms, h.event(4 * gi + 1), h.event(4 * gi + 2)));
+        HIPC(hipEventElapsedTime(&ms, h.event(half, 4 * gi + 1), h.event(half, 4 * gi + 2)));
         stage[1] = ms;
-        HIPC(hipEventElapsedTime(&ms, h.event(4 * gi + 2), h.event(4 * gi + 3)));
+        HIPC(hipEventElapsedTime(&ms, h.event(half, 4 * gi + 2), h.event(half, 4 * gi + 3)));
         stage[3] = ms;
-        for (int s = 0; s < 4; ++s) h.kernel_ms[s] += stage[s];
+        for (int s = 0; s < 4; ++s) km[s] += stage[s];
         for (uint32_t r = first; r < first + n; ++r)
-            for (int s = 0; s < 4; ++s) h.stage_ms_per_template[4 * r + s] = stage[s] / n;
+            for (int s = 0; s < 4; ++s) per[4 * r + s] = stage[s] / n;
     }
 }
 
-void read_slot(brp_hip& h, uint32_t slot, std::vector<brp_candidate>& out) {
-    require(h.ran, "run() has not been called");
-    require(slot < h.templates.size(), "candidate slot outside executed batch");
-    const uint32_t* rec = h.staging_host + size_t(slot) * (h.keep + 1) * 4;
+// Enqueues the templates of `half` (parameters up, kernels, packed results down) and
+// records its completion event; nothing waits.
+void launch(brp_hip& h, int half) {
+    const uint32_t count = uint32_t(h.templates[half].size());
+    const size_t base = h.slot_base(half);
+    for (uint32_t i = 0; i < count; ++i) {
+        h.params_host[base + i] = h.templates[half][i];
+        h.params_host[base + i].result_slot = uint32_t(base + i);
+    }
+    const auto submit0 = std::chrono::steady_clock::now();
+    HIPC(hipMemcpyAsync(h.params_dev + base, h.params_host + base, sizeof(brp_params) * count,
+                        hipMemcpyHostToDevice, h.stream));
+    if (h.opt.engine == BRP_ENGINE_PLAIN) {
+        require(half == 0, "the plain engine runs one batch at a time");
+        if (h.opt.snapshot)
+            HIPC(hipMemsetAsync(h.power_buf + h.base.n / 2, 0, sizeof(float) * 5ull * h.base.fundamental, h.stream));
+        for (uint32_t r = 0; r < count; ++r) enqueue_plain(h, r);
+        brp_engine::pack_results(h.stream, h.results, h.slot_words, count, h.keep, h.staging_dev);
+    } else {
+        if (h.opt.snapshot) {
+            require(half == 0, "snapshots are kept for one batch at a time");
+            HIPC(hipMemsetAsync(h.fa.snap_summed, 0, sizeof(float) * size_t(count) * 5 * h.fa.snap_f, h.stream));
+            HIPC(hipMemsetAsync(h.fa.snap_X, 0, sizeof(float2) * size_t(count) * (h.shape.m + 1), h.stream));
+        }
+        run_fused(h, half, count);
+    }
+    h.submit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - submit0).count();
+    HIPC(hipMemcpyAsync(h.staging_host + base * (h.keep + 1) * 4, h.staging_dev + base * (h.keep + 1) * 4,
+                        sizeof(uint32_t) * size_t(count) * (h.keep + 1) * 4, hipMemcpyDeviceToHost, h.stream));
+    if (!h.done[half]) HIPC(hipEventCreateWithFlags(&h.done[half], hipEventDisableTiming));
+    HIPC(hipEventRecord(h.done[half], h.stream));
+    h.ran[half] = false;
+}
+
+void finish(brp_hip& h, int half) {
+    HIPC(hipEventSynchronize(h.done[half]));
+    h.ran[half] = true;
+    const uint32_t count = uint32_t(h.templates[half].size());
+    fill_timings(h, half, count);
+    for (int s = 0; s < 4; ++s) h.total_ms[s] += h.kernel_ms[half][s];
+    h.total_launches += h.opt.engine == BRP_ENGINE_PLAIN ? count : (count + h.group - 1) / h.group;
+}
+
+void read_slot(brp_hip& h, int half, uint32_t slot, std::vector<brp_candidate>& out) {
+    require(h.ran[half], "run() has not been called");
+    require(slot < h.templates[half].size(), "candidate slot outside executed batch");
+    const size_t global = h.slot_base(half) + slot;
+    const uint32_t* rec = h.staging_host + global * (h.keep + 1) * 4;
     const uint32_t count = rec[0], flags = rec[1];
-    const brp_params& p = h.templates[slot];
+    const brp_params& p = h.templates[half][slot];
     if (flags || count > p.capacity)
         throw Failure(BRP_E_OVERFLOW, "GPU invalid resample index or candidate overflow; no valid result");
     out.resize(count);
     if (count <= h.keep) {
         std::memcpy(out.data(), rec + 4, sizeof(brp_candidate) * count);
     } else {
-        HIPC(hipMemcpy(out.data(), h.results + size_t(slot) * h.slot_words + 4, sizeof(brp_candidate) * count,
+        HIPC(hipMemcpy(out.data(), h.results + global * h.slot_words + 4, sizeof(brp_candidate) * count,
                        hipMemcpyDeviceToHost));
     }
     std::sort(out.begin(), out.end(), [](const brp_candidate& a, const brp_candidate& b) {
@@ -400,15 +457,17 @@ int brp_hip_create(const brp_params* p, const float* input, size_t count, const 
         for (int i = 0; i < 65; ++i) lut[i] = make_float2(brp_lut::kTable[2 * i], brp_lut::kTable[2 * i + 1]);
         h->lut = h->upload(lut);
         h->slot_words = 4 + 4 * p->capacity;
-        h->results = h->alloc<uint32_t>(size_t(opt->batch) * h->slot_words);
-        HIPC(hipMemset(h->results, 0, sizeof(uint32_t) * size_t(opt->batch) * h->slot_words));
-        h->staging_dev = h->alloc<uint32_t>(size_t(opt->batch) * (h->keep + 1) * 4);
-        HIPC(hipHostMalloc(reinterpret_cast<void**>(&h->staging_host), sizeof(uint32_t) * size_t(opt->batch) * (h->keep + 1) * 4, hipHostMallocDefault));
-        h->params_dev = h->alloc<brp_params>(opt->batch);
-        HIPC(hipHostMalloc(reinterpret_cast<void**>(&h->params_host), sizeof(brp_params) * opt->batch, hipHostMallocDefault));
+        const size_t slots = 2 * size_t(opt->batch);  // two halves
+        h->results = h->alloc<uint32_t>(slots * h->slot_words);
+        HIPC(hipMemset(h->results, 0, sizeof(uint32_t) * slots * h->slot_words));
+        h->staging_dev = h->alloc<uint32_t>(slots * (h->keep + 1) * 4);
+        HIPC(hipHostMalloc(reinterpret_cast<void**>(&h->staging_host), sizeof(uint32_t) * slots * (h->keep + 1) * 4,
+                           hipHostMallocDefault));
+        h->params_dev = h->alloc<brp_params>(slots);
+        HIPC(hipHostMalloc(reinterpret_cast<void**>(&h->params_host), sizeof(brp_params) * slots, hipHostMallocDefault));
         if (opt->engine == BRP_ENGINE_PLAIN) create_plain(*h);
         else create_fused(*h);
-        h->templates.assign(1, *p);
+        h->templates[0].assign(1, *p);
         *out = h.release();
     });
 }
@@ -422,50 +481,26 @@ int brp_hip_set_templates(brp_hip* h, const brp_params* p, size_t count) {
         require(h && p, "null argument");
         require(count && count <= h->opt.batch, "template batch exceeds retained result capacity");
         for (size_t i = 0; i < count; ++i) check_template(h->base, p[i]);
-        h->templates.assign(p, p + count);
-        h->ran = false;
+        h->templates[0].assign(p, p + count);
+        h->ran[0] = false;
     });
 }
 
 int brp_hip_run(brp_hip* h, brp_timings* t, double* template_stage_ms) {
     return guarded([&] {
         require(h, "null handle");
-        const uint32_t count = uint32_t(h->templates.size());
         const auto start = std::chrono::steady_clock::now();
         const double cpu0 = process_cpu_ms();
-        for (uint32_t i = 0; i < count; ++i) {
-            h->params_host[i] = h->templates[i];
-            h->params_host[i].result_slot = i;
-        }
-        const auto submit0 = std::chrono::steady_clock::now();
-        HIPC(hipMemcpyAsync(h->params_dev, h->params_host, sizeof(brp_params) * count, hipMemcpyHostToDevice, h->stream));
-        if (h->opt.engine == BRP_ENGINE_PLAIN) {
-            if (h->opt.snapshot) {
-                HIPC(hipMemsetAsync(h->power_buf + h->base.n / 2, 0, sizeof(float) * 5ull * h->base.fundamental, h->stream));
-            }
-            for (uint32_t r = 0; r < count; ++r) enqueue_plain(*h, r);
-            brp_engine::pack_results(h->stream, h->results, h->slot_words, count, h->keep, h->staging_dev);
-        } else {
-            if (h->opt.snapshot) {
-                HIPC(hipMemsetAsync(h->fa.snap_summed, 0, sizeof(float) * size_t(count) * 5 * h->fa.snap_f, h->stream));
-                HIPC(hipMemsetAsync(h->fa.snap_X, 0, sizeof(float2) * size_t(count) * (h->shape.m + 1), h->stream));
-            }
-            run_fused(*h, count);
-        }
-        const double submit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - submit0).count();
-        HIPC(hipMemcpyAsync(h->staging_host, h->staging_dev, sizeof(uint32_t) * size_t(count) * (h->keep + 1) * 4,
-                            hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
-        h->ran = true;
-        fill_timings(*h, count);
+        launch(*h, 0);
+        finish(*h, 0);
         if (t) {
-            t->submit_ms = submit_ms;
-            for (int s = 0; s < 4; ++s) t->stage_ms[s] = h->kernel_ms[s];
+            t->submit_ms = h->submit_ms;
+            for (int s = 0; s < 4; ++s) t->stage_ms[s] = h->kernel_ms[0][s];
             t->cpu_ms = process_cpu_ms() - cpu0;
             t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
         }
         if (template_stage_ms)
-            std::copy(h->stage_ms_per_template.begin(), h->stage_ms_per_template.end(), template_stage_ms);
+            std::copy(h->stage_ms_per_template[0].begin(), h->stage_ms_per_template[0].end(), template_stage_ms);
     });
 }
 
@@ -473,7 +508,7 @@ int brp_hip_candidates(brp_hip* h, uint32_t slot, brp_candidate* out, size_t cap
     return guarded([&] {
         require(h && count, "null argument");
         std::vector<brp_candidate> v;
-        read_slot(*h, slot, v);
+        read_slot(*h, 0, slot, v);
         *count = v.size();
         require(v.size() <= cap || !out, "candidate buffer too small");
         if (out) std::copy(v.begin(), v.end(), out);
@@ -483,14 +518,14 @@ int brp_hip_candidates(brp_hip* h, uint32_t slot, brp_candidate* out, size_t cap
 int brp_hip_snapshot(brp_hip* h, uint32_t slot, float* resampled, int32_t* index, float* fft, float* power,
                      float* summed) {
     return guarded([&] {
-        require(h && h->ran, "run() has not been called");
-        require(slot < h->templates.size(), "snapshot slot outside executed batch");
-        const brp_params& p = h->templates[slot];
+        require(h && h->ran[0], "run() has not been called");
+        require(slot < h->templates[0].size(), "snapshot slot outside executed batch");
+        const brp_params& p = h->templates[0][slot];
         const uint32_t n = p.n, half = n / 2;
         if (h->opt.engine == BRP_ENGINE_PLAIN) {
             // The plain engine runs a batch sequentially on shared buffers (like the reference),
             // so its arrays hold the last template of the batch.
-            require(slot + 1 == h->templates.size(), "plain engine snapshots hold the last template of the batch");
+            require(slot + 1 == h->templates[0].size(), "plain engine snapshots hold the last template of the batch");
             if (resampled) {
                 HIPC(hipMemcpy(resampled, h->pb.raw, sizeof(float) * p.steps, hipMemcpyDeviceToHost));
                 float mean = 0;
@@ -560,10 +595,12 @@ size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap) {
     return text.size();
 }
 
-int brp_hip_kernel_ms(brp_hip* h, double out[4]) {
+int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches) {
     return guarded([&] {
-        require(h && out && h->ran, "run() has not been called");
-        std::copy(h->kernel_ms, h->kernel_ms + 4, out);
+        require(h && out, "null argument");
+        for (int s = 0; s < 4; ++s) { out[s] = h->total_ms[s]; h->total_ms[s] = 0; }
+        if (launches) *launches = h->total_launches;
+        h->total_launches = 0;
     });
 }
 
@@ -614,27 +651,41 @@ int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint3
         for (size_t i = 0; i < count; ++i)
             if (int rc = brp_search_parameters(h->base.n, samples, sample_us, &orbits[i], f0, false_alarm, window, &base[i]))
                 throw Failure(rc, brp_last_error());
+        for (const auto& p : base) check_template(h->base, p);
         const size_t batch = h->opt.batch;
+        const size_t batches = (count + batch - 1) / batch;
+        // The fused engine keeps two batches in flight: batch b + 1 is enqueued with the
+        // thresholds of the toplist before batch b is retained.  Those are never higher
+        // than the thresholds batch b + 1 would see one batch at a time, so its candidates
+        // are a superset, and retain() with each template's current thresholds keeps
+        // exactly the same rows.  The plain engine runs batch by batch, as the reference.
+        const bool overlap = h->opt.engine == BRP_ENGINE_FUSED && !h->opt.snapshot;
+        auto enqueue = [&](size_t b) {
+            const int half = overlap ? int(b & 1) : 0;
+            const size_t first = b * batch, n = std::min(batch, count - first);
+            h->templates[half].assign(base.begin() + first, base.begin() + first + n);
+            for (auto& p : h->templates[half]) brp_toplist_thresholds(top, &p);
+            launch(*h, half);
+        };
         double total_gpu = 0;
-        std::vector<brp_params> group;
         std::vector<brp_candidate> cand;
-        for (size_t index = 0; index < count; index += batch) {
-            const size_t n = std::min(batch, count - index);
-            group.assign(base.begin() + index, base.begin() + index + n);
-            for (auto& p : group) brp_toplist_thresholds(top, &p);  // frozen at the batch start
-            if (int rc = brp_hip_set_templates(h, group.data(), n)) throw Failure(rc, brp_last_error());
-            brp_timings t{};
-            if (int rc = brp_hip_run(h, &t, nullptr)) throw Failure(rc, brp_last_error());
-            for (int s = 0; s < 4; ++s) total_gpu += t.stage_ms[s];
+        if (batches) enqueue(0);
+        for (size_t b = 0; b < batches; ++b) {
+            const int half = overlap ? int(b & 1) : 0;
+            if (overlap && b + 1 < batches) enqueue(b + 1);
+            finish(*h, half);
+            for (int s = 0; s < 4; ++s) total_gpu += h->kernel_ms[half][s];
+            const size_t first = b * batch, n = h->templates[half].size();
             for (size_t k = 0; k < n; ++k) {
-                brp_params now = base[index + k];
+                brp_params now = base[first + k];
                 brp_toplist_thresholds(top, &now);  // the toplist as of this template
-                read_slot(*h, uint32_t(k), cand);
-                if (int rc = brp_toplist_retain(top, cand.data(), cand.size(), &orbits[index + k], &now))
+                read_slot(*h, half, uint32_t(k), cand);
+                if (int rc = brp_toplist_retain(top, cand.data(), cand.size(), &orbits[first + k], &now))
                     throw Failure(rc, brp_last_error());
-                if (templates_out) templates_out[index + k] = now;
-                if (cand_count) cand_count[index + k] = uint32_t(cand.size());
+                if (templates_out) templates_out[first + k] = now;
+                if (cand_count) cand_count[first + k] = uint32_t(cand.size());
             }
+            if (!overlap && b + 1 < batches) enqueue(b + 1);
         }
         if (gpu_ms) *gpu_ms = total_gpu;
     });

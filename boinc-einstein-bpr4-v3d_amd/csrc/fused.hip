@@ -162,62 +162,80 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
     if (t < 65) lut[t] = a.lut[t];
     __syncthreads();
 
-    // Indices of the thread's 2E samples, in register order.
-    int src[2 * E];
-#pragma unroll
-    for (int u = 0; u < E / R0; ++u) {
-        const int b = t + T * u;
+    // Element e of the thread (butterfly u = e / R0, input q = e % R0) is the packed pair
+    // z[m] = (y[2m], y[2m+1]) with m = r0 + s + R c, c = j + q C/R0.
+    auto element_m = [&](int e) -> uint32_t {
+        const int b = t + T * (e / R0);
         const int s = b % S, j = b / S;
-#pragma unroll
-        for (int q = 0; q < R0; ++q) {
-            const uint32_t m = r0 + s + R * uint32_t(j + q * (C / R0));
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const uint32_t i = 2 * m + h;
-                int jj;
-                if constexpr (F64) {
-                    const double tau_dt = double(p.tau) / double(p.dt);
-                    const double s0_dt = double(p.tau) * sin(double(p.psi)) / double(p.dt);
-                    jj = source_index_fp64(i, tau_dt, double(p.omega), double(p.psi), double(p.dt), s0_dt);
-                } else {
-                    jj = source_index(i, o, lut);
-                }
-                src[2 * (u * R0 + q) + h] = jj;
-            }
+        return r0 + s + __umul24(R, uint32_t(j + (e % R0) * (C / R0)));  // R, c < 2^24
+    };
+    auto index_of = [&](uint32_t i) -> int {
+        if constexpr (F64) {
+            const double tau_dt = double(p.tau) / double(p.dt);
+            const double s0_dt = double(p.tau) * sin_f64(double(p.psi)) / double(p.dt);
+            return source_index_fp64(i, tau_dt, double(p.omega), double(p.psi), double(p.dt), s0_dt);
+        } else {
+            return source_index(i, o, lut);
         }
-    }
-    float y[2 * E];
-#pragma unroll
-    for (int e = 0; e < 2 * E; ++e) y[e] = a.x[min(max(src[e], 0), int(U) - 1)];
+    };
 
+    // In chunks of CH elements: every index of the chunk, then every load, then the
+    // masking.  The two samples of a pair come from source indices j0 <= j1 <= j0 + 2 (the
+    // orbit moves the index by far less than one sample per sample), so one 12-byte load
+    // at b = clamp(j0, 0, U - 3) serves both; a pair outside its window (only at the ends
+    // of the observation) takes two clamped single loads.
+    constexpr int CH = F64 ? 2 : (E < 8 ? E : 8);
+    static_assert(E % CH == 0, "chunking");
+    struct Trio { float v[3]; };
     double acc = 0.0;
     bool bad = false;
     cf v[E];
 #pragma unroll
-    for (int u = 0; u < E / R0; ++u) {
-        const int b = t + T * u;
-        const int s = b % S, j = b / S;
+    for (int c0 = 0; c0 < E; c0 += CH) {
+        int src[2 * CH];
 #pragma unroll
-        for (int q = 0; q < R0; ++q) {
-            const uint32_t m = r0 + s + R * uint32_t(j + q * (C / R0));
+        for (int e = 0; e < CH; ++e) {
+            const uint32_t m = element_m(c0 + e);
+            src[2 * e] = index_of(2 * m);
+            src[2 * e + 1] = index_of(2 * m + 1);
+        }
+        float y[2 * CH];
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            const int j0 = src[2 * e], j1 = src[2 * e + 1];
+            const int b = min(max(j0, 0), int(U) - 3);
+            Trio w;
+            __builtin_memcpy(&w, a.x + b, sizeof(Trio));
+            const uint32_t d0 = uint32_t(j0 - b), d1 = uint32_t(j1 - b);
+            if (d0 <= 2u && d1 <= 2u) {
+                y[2 * e] = d0 == 0u ? w.v[0] : (d0 == 1u ? w.v[1] : w.v[2]);
+                y[2 * e + 1] = d1 == 0u ? w.v[0] : (d1 == 1u ? w.v[1] : w.v[2]);
+            } else {
+                y[2 * e] = a.x[min(max(j0, 0), int(U) - 1)];
+                y[2 * e + 1] = a.x[min(max(j1, 0), int(U) - 1)];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            const uint32_t m = element_m(c0 + e);
             float pair[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const uint32_t i = 2 * m + h;
-                const int e = 2 * (u * R0 + q) + h;
+                const int jj = src[2 * e + h];
                 const bool in_series = i < steps;
-                const bool inside = src[e] >= 0 && uint32_t(src[e]) < U;
+                const bool inside = jj >= 0 && uint32_t(jj) < U;
                 bad |= in_series && !inside;
-                pair[h] = (in_series && inside) ? y[e] : 0.0f;
+                pair[h] = (in_series && inside) ? y[2 * e + h] : 0.0f;
                 acc += double(pair[h]);
                 if constexpr (SNAP) {
                     if (in_series) {
                         a.snap_resampled[size_t(slot) * sh.n + i] = pair[h];
-                        a.snap_index[size_t(slot) * sh.n + i] = src[e];
+                        a.snap_index[size_t(slot) * sh.n + i] = jj;
                     }
                 }
             }
-            v[u * R0 + q] = cf{pair[0], pair[1]};
+            v[c0 + e] = cf{pair[0], pair[1]};
         }
     }
     fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
@@ -231,7 +249,7 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
 #pragma unroll
         for (int q = 0; q < RL; ++q) {
             const uint32_t k2 = uint32_t(j + q * (C / RL));
-            store(Y + size_t(r) * C + k2, mul(v[u * RL + q], twiddle2(a.twm_lo, a.twm_hi, r * k2)));
+            store(Y + size_t(r) * C + k2, mul(v[u * RL + q], twiddle2(a.twm_lo, a.twm_hi, __umul24(r, k2))));
         }
     }
 
@@ -313,13 +331,24 @@ __global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
     const uint32_t steps = p.steps;
     const float pad_len = float(N - steps);
     const float parity = (steps & 1u) ? -1.0f : 1.0f;  // W_N^{M steps} = (-1)^steps
-    const bool n_pow2 = (N & (N - 1)) == 0;
-    const uint32_t lg = __builtin_ctz(N);  // N = 2^lg or 3 * 2^lg
+    // Pad-correction twiddle W_N^{k steps}: a thread's pairs share the column (T is a
+    // multiple of S) and step k by T/S rows, so e(k) = k steps mod N advances by the
+    // constant D = (T/S) C steps mod N.  W_N^{e} starts from one table lookup and is then
+    // multiplied by W_N^D per pair (the correction term tolerates the few-ulp drift).
+    static_assert(T % S == 0, "pair mapping");
+    auto mod_n = [&](uint64_t x) -> uint32_t {
+        return uint32_t(x % uint64_t(N));
+    };
+    const uint32_t steps_n = steps % N;
+    const uint32_t k_first = uint32_t(t / S) * C + base + uint32_t(t % S);
+    cf we = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(k_first) * steps_n));
+    const cf wstep = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(T / S) * C % N * steps_n));
 #pragma unroll
     for (int u = 0; u < R * S / T; ++u) {
         const int pi = t + T * u;
         const int s = pi % S;
         const uint32_t k1 = uint32_t(pi / S);
+        if (u) we = mul(we, wstep);
         if (middle && s != 0) continue;
         const uint32_t col = base + uint32_t(s);
         const bool self = col == 0 || col == C / 2;
@@ -335,18 +364,6 @@ __global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
             xk.x += mu * pad_len;
             xm.x += mu * (parity - 1.0f) * 0.5f;  // U[M] = ((-1)^steps - 1) / 2
         } else {
-            // e = k steps mod N.  N = 2^lg: the 32-bit product wraps modulo a multiple of N.
-            // N = 3 * 2^lg: x = hi 2^lg + lo with hi < 2^25, and x mod N = (hi mod 3) 2^lg + lo.
-            uint32_t e;
-            if (n_pow2) {
-                e = (k * steps) & (N - 1);
-            } else {
-                const uint64_t x = uint64_t(k) * steps;
-                const uint32_t hi = uint32_t(x >> lg);
-                const uint32_t third = __umulhi(hi, 0xAAAAAAABu) >> 1;
-                e = ((hi - 3u * third) << lg) | (uint32_t(x) & ((1u << lg) - 1u));
-            }
-            const cf we = twiddle2(a.twn_lo, a.twn_hi, e);
             // cot(pi k / N) and tan(pi k / N) = cot(pi (M - k) / N) from W_N^k = (cos, -sin),
             // each in the half-angle form without cancellation for its half of (0, pi).
             const bool front = W.x >= 0.0f;
@@ -370,21 +387,20 @@ __global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
 
 // ---------------------------------------------------------------- harmonic --
 // Bins i < H < 2^24, so every index of the dyadic sums, (i k + 8) >> 4, is one 24-bit
-// multiply-add and a shift, and the loads use 32-bit offsets from the template's power.
-__device__ __forceinline__ void dyadic_sums24(const float* __restrict__ P, uint32_t i, float s[4]) {
-#define BRP_AT(k) P[(__umul24(i, (k)) + 8u) >> 4]
-    float v = P[i];
-    v += BRP_AT(8u); s[0] = v;
-    v += BRP_AT(12u) + BRP_AT(4u); s[1] = v;
-    v += BRP_AT(14u) + BRP_AT(10u) + BRP_AT(6u) + BRP_AT(2u); s[2] = v;
-    v += BRP_AT(15u) + BRP_AT(13u) + BRP_AT(11u) + BRP_AT(9u) + BRP_AT(7u) + BRP_AT(5u) + BRP_AT(3u) + BRP_AT(1u);
-    s[3] = v;
-#undef BRP_AT
-}
+// multiply-add and a shift.  A thread owns the bins i, i + 256, i + 512, i + 768 of a
+// tile; bin i + 256 u reads index (i k + 8) >> 4 + 16 k u, so the four bins share one
+// set of index computations and differ only by constant offsets.
+constexpr int kHarmThreads = 256;
+constexpr int kPerThread = kTile / kHarmThreads;
 
-template <int T, bool SNAP>
-__global__ __launch_bounds__(T) void k_harmonic(FusedArgs a, FusedShape sh, uint32_t tiles) {
-    __shared__ float sums[4][kTile + 8];
+// LDS slot of tile bin d (0 <= d < kTile + 8) in a row of sums: one pad word per 16, so
+// the interval reads of level h (lanes 2^h bins apart) spread over the banks.
+__device__ __forceinline__ int sum_at(int d) { return d + (d >> 4); }
+constexpr int kSumRow = kTile + 8 + (kTile + 8) / 16 + 1;
+
+template <bool SNAP>
+__global__ __launch_bounds__(kHarmThreads) void k_harmonic(FusedArgs a, FusedShape sh, uint32_t tiles) {
+    __shared__ float sums[4][kSumRow];
     const int t = threadIdx.x;
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t g = item / tiles, tile = item % tiles;
@@ -397,17 +413,43 @@ __global__ __launch_bounds__(T) void k_harmonic(FusedArgs a, FusedShape sh, uint
     uint32_t* res = a.results + size_t(slot) * a.slot_words;
     float* snap = SNAP ? a.snap_summed + size_t(slot) * 5 * a.snap_f : nullptr;
 
-    // Phase A: dyadic sums of bins i0 - 8 .. i0 + kTile - 1 into LDS (-inf outside
-    // [W, H)); the level-0 test of the fundamentals j = i in [i0, i0 + kTile) rides along.
-    for (int idx = t; idx < kTile + 8; idx += T) {
-        const int i = i0 - 8 + idx;
-        float s[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        if (i >= W && i < H) dyadic_sums24(P, uint32_t(i), s);
-        sums[0][idx] = s[0]; sums[1][idx] = s[1]; sums[2][idx] = s[2]; sums[3][idx] = s[3];
-        if (idx >= 8 && i < F) {
-            const float v = P[i];
-            if constexpr (SNAP) snap[i] = v;
-            if (i >= W && v > thr0) append_candidate(res, p.capacity, uint32_t(i), 0, v);
+    // Phase A: dyadic sums of the tile's bins i0 .. i0 + kTile - 1 (LDS slots 8 ..) and
+    // of the 8 bins before it (slots 0 .. 7), -inf outside [W, H); level-0 candidates.
+    {
+        const uint32_t ib = uint32_t(i0 + t);
+        uint32_t idx[16];
+#pragma unroll
+        for (int k = 1; k <= 16; ++k) idx[k - 1] = (__umul24(ib, uint32_t(k)) + 8u) >> 4;
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) {
+            const int i = i0 + t + kHarmThreads * u;
+            float s[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            if (i >= W && i < H) {
+                auto at = [&](int k) { return P[idx[k - 1] + uint32_t(16 * k * u)]; };
+                float v = at(16);
+                v += at(8); s[0] = v;
+                v += at(12) + at(4); s[1] = v;
+                v += at(14) + at(10) + at(6) + at(2); s[2] = v;
+                v += at(15) + at(13) + at(11) + at(9) + at(7) + at(5) + at(3) + at(1); s[3] = v;
+            }
+            const int d = sum_at(8 + t + kHarmThreads * u);
+            sums[0][d] = s[0]; sums[1][d] = s[1]; sums[2][d] = s[2]; sums[3][d] = s[3];
+            if (i < F) {
+                const float v0 = P[i];
+                if constexpr (SNAP) snap[i] = v0;
+                if (i >= W && v0 > thr0) append_candidate(res, p.capacity, uint32_t(i), 0, v0);
+            }
+        }
+        if (t < 8) {
+            const int i = i0 - 8 + t;
+            float s[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            if (i >= W && i < H) {
+                float full[5];
+                dyadic_sums(P, uint32_t(i), full);
+                s[0] = full[1]; s[1] = full[2]; s[2] = full[3]; s[3] = full[4];
+            }
+            const int d = sum_at(t);
+            sums[0][d] = s[0]; sums[1][d] = s[1]; sums[2][d] = s[2]; sums[3][d] = s[3];
         }
     }
     __syncthreads();
@@ -419,7 +461,7 @@ __global__ __launch_bounds__(T) void k_harmonic(FusedArgs a, FusedShape sh, uint
         const int groups = kTile >> h;
         const int j0 = i0 >> h;
         const float thr = p.threshold[h];
-        for (int gi = t; gi < groups; gi += T) {
+        for (int gi = t; gi < groups; gi += kHarmThreads) {
             const int j = j0 + gi;
             if (j >= F) continue;
             const int begin = max(W, j * width - width / 2);
@@ -428,9 +470,9 @@ __global__ __launch_bounds__(T) void k_harmonic(FusedArgs a, FusedShape sh, uint
             if (begin < end) {
                 const float* row = sums[h - 1];
                 const int lo = begin - i0 + 8, hi = end - i0 + 8;
-                const float first = row[lo];
+                const float first = row[sum_at(lo)];
                 float best = first;
-                for (int i = lo + 1; i < hi; ++i) best = fmaxf(best, row[i]);
+                for (int d = lo + 1; d < hi; ++d) best = fmaxf(best, row[sum_at(d)]);
                 value = best > thr ? best : first;
             }
             if constexpr (SNAP) snap[size_t(h) * a.snap_f + j] = value;
@@ -534,8 +576,8 @@ void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
 }
 
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles) {
-    if (a.snap_summed) k_harmonic<kThreads, true><<<tiles * a.count, kThreads, 0, st>>>(a, s, tiles);
-    else k_harmonic<kThreads, false><<<tiles * a.count, kThreads, 0, st>>>(a, s, tiles);
+    if (a.snap_summed) k_harmonic<true><<<tiles * a.count, kHarmThreads, 0, st>>>(a, s, tiles);
+    else k_harmonic<false><<<tiles * a.count, kHarmThreads, 0, st>>>(a, s, tiles);
 }
 
 void copy16(hipStream_t st, const void* src, void* dst, size_t bytes) {

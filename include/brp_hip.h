@@ -170,9 +170,12 @@ int brp_hip_snapshot(brp_hip* h, uint32_t slot, float* resampled, int32_t* index
 size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap);
 uint64_t brp_hip_allocated_bytes(const brp_hip* h);
 size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap);
-/* Per-kernel device times of the last run (ms), summed over the batch, as measured
- * with HIP events on the launch stream: pass1/resample, fft, power, harmonic. */
-int brp_hip_kernel_ms(brp_hip* h, double out[4]);
+/* Per-stage device times (ms) accumulated over every run since the previous call of this
+ * function (which resets them), measured with HIP events on the launch stream:
+ * resample/pass 1, fft/pass 2, power, harmonic.  *launches (may be NULL) receives the
+ * number of launch sequences timed: groups of templates (fused engine) or templates
+ * (plain engine). */
+int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches);
 /* Measurement aid with no reference counterpart: copies `bytes` between two device
  * buffers `reps` times with a plain streaming kernel (brp_copy) and returns the mean
  * GB/s; used to calibrate hardware byte counters against a transfer of known size. */
