@@ -45,7 +45,10 @@ def main() -> int:
     ap.add_argument("--templates", type=int, default=512)
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--batch", type=int, default=64, help="templates per run (toplist thresholds frozen per run)")
-    ap.add_argument("--group", type=int, default=8, help="templates per kernel launch")
+    ap.add_argument("--group", type=int, default=8, help="templates per kernel launch (without --no-tune: tuned)")
+    ap.add_argument("--split-c", type=int, default=0, help="pass-1 FFT length (without --no-tune: tuned)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner, use --group / --split-c")
+    ap.add_argument("--tune-templates", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--plain-templates", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -72,9 +75,21 @@ def main() -> int:
     copy_gbps = brp.copy_bandwidth(256 << 20, 20)
     log(f"[bench] streaming copy of 256 MiB: {copy_gbps:.1f} GB/s")
 
+    # ---------------------------------------------------------------- autotune --
+    # Four-step split (LDS tiles, radix sequence) and templates per launch, timed on a
+    # slice of the bank.
+    tuned = None
+    if not args.no_tune:
+        t2 = time.time()
+        best, report = brp.autotune(w.templates[:args.tune_templates], w.x,
+                                    {"graph": int(not args.no_graph)}, reps=3)
+        args.group, args.split_c = best["group"], best["split_c"]
+        tuned = {"picked": best, "candidates": report, "seconds": round(time.time() - t2, 2)}
+        log(f"[bench] autotune picked {best} in {tuned['seconds']} s: {report}")
+
     # ---------------------------------------------------------------- batched --
     g = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
-                graph=not args.no_graph, group=args.group)
+                graph=not args.no_graph, group=args.group, split_c=args.split_c)
     log("[bench]", g.device_info())
 
     def bank_pass():
@@ -170,7 +185,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"bank of {args.templates} orbital templates x N={n} series (bank semantics)",
                    "templates": args.templates, "n": n, "batch": args.batch, "group": args.group,
-                   "graph": not args.no_graph},
+                   "split_c": args.split_c, "graph": not args.no_graph},
+        "autotune": tuned,
         "plain_templates_per_s": round(plain_tps, 3) if plain_tps else None,
         "batched_over_plain": round(templates_per_s / plain_tps, 3) if plain_tps else None,
         "toplist_identical": identical,

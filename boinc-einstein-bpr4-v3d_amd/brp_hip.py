@@ -119,6 +119,7 @@ def load() -> ctypes.CDLL:
         "brp_hip_profile_counters": (sz, [ctypes.c_void_p, ctypes.c_char_p, sz]),
         "brp_hip_kernel_ms": (i32, [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_uint64)]),
         "brp_hip_copy_bandwidth": (i32, [sz, i32, P(ctypes.c_double)]),
+        "brp_hip_autotune": (i32, [P(Params), sz, fp, sz, P(Options), u32, P(Options), ctypes.c_char_p, sz]),
         "brp_hip_bank_search": (i32, [ctypes.c_void_p, P(Orbit), sz, u32, f64, f64, f64, u32, ctypes.c_void_p,
                                       P(Params), P(u32), P(ctypes.c_double)]),
     }
@@ -183,6 +184,23 @@ def synthetic_bank(seed: int, count: int, pmin: float = 660.0, pmax: float = 100
     arr = (Orbit * count)()
     _check(load().brp_synthetic_bank(seed, count, pmin, pmax, tau_max, arr))
     return arr
+
+
+def autotune(templates, x: np.ndarray, batch_options: "dict | None" = None, reps: int = 3):
+    """Times every compiled four-step split and templates-per-launch value on these
+    templates; returns (best option dict, list of candidate reports)."""
+    import json
+    arr = (Params * len(templates))(*templates)
+    xs = np.ascontiguousarray(x, dtype=np.float32)
+    kw = dict(workgroup=256, batch=len(templates), packed=1, engine=ENGINE_FUSED, graph=1)
+    kw.update(batch_options or {})
+    base = Options(**{k: v for k, v in kw.items() if k in dict(Options._fields_)})
+    best = Options()
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(load().brp_hip_autotune(arr, len(arr), _fptr(xs), xs.size, ctypes.byref(base), reps, ctypes.byref(best),
+                                   buf, len(buf)))
+    report = [json.loads(line) for line in buf.value.decode().splitlines() if line.strip()]
+    return {"split_c": best.split_c, "group": best.group}, report
 
 
 def copy_bandwidth(nbytes: int, reps: int = 10) -> float:

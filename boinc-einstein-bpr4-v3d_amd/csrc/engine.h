@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "brp_hip.h"
 
 namespace brp_engine {
@@ -71,6 +73,8 @@ struct FusedArgs {
 };
 
 bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* shape);
+// Every compiled shape of series length n, the default first.
+std::vector<FusedShape> fused_shapes(uint32_t n);
 // One launch each covers all `count` templates of the group.
 void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a);
 void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a);

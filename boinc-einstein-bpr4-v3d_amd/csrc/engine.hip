@@ -604,6 +604,61 @@ int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches) {
     });
 }
 
+int brp_hip_autotune(const brp_params* templates, size_t count, const float* input, size_t input_count,
+                     const brp_options* base, uint32_t reps, brp_options* best, char* report, size_t cap) {
+    return guarded([&] {
+        require(templates && count && input && base && best, "null argument");
+        require(reps > 0 && reps <= 1000, "invalid repetitions");
+        const uint32_t n = templates[0].n;
+        const std::vector<brp_engine::FusedShape> shapes = brp_engine::fused_shapes(n);
+        require(!shapes.empty(), "fused engine: unsupported series length");
+        std::ostringstream log;
+        double best_us = 1e300;
+        *best = *base;
+        for (const auto& s : shapes) {
+            for (uint32_t group : {2u, 4u, 8u, 16u}) {
+                if (group > count) continue;
+                brp_options o = *base;
+                o.engine = BRP_ENGINE_FUSED;
+                o.packed = 1;
+                o.snapshot = 0;
+                o.batch = uint32_t(count);
+                o.split_c = s.c;
+                o.group = group;
+                brp_hip* h = nullptr;
+                if (int rc = brp_hip_create(&templates[0], input, input_count, &o, &h)) throw Failure(rc, brp_last_error());
+                std::unique_ptr<brp_hip> owner(h);
+                if (int rc = brp_hip_set_templates(h, templates, count)) throw Failure(rc, brp_last_error());
+                launch(*h, 0);
+                finish(*h, 0);  // warm-up (and graph capture)
+                std::vector<double> us;
+                for (uint32_t r = 0; r < reps; ++r) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    launch(*h, 0);
+                    finish(*h, 0);
+                    us.push_back(std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() /
+                                 double(count));
+                }
+                std::sort(us.begin(), us.end());
+                const double med = us[us.size() / 2];
+                log << "{\"C\":" << s.c << ",\"R\":" << s.r << ",\"S1\":" << s.s1 << ",\"S2\":" << s.s2
+                    << ",\"group\":" << group << ",\"us_per_template\":" << med << "}\n";
+                if (med < best_us) {
+                    best_us = med;
+                    best->split_c = s.c;
+                    best->group = group;
+                }
+            }
+        }
+        const std::string text = log.str();
+        if (report && cap) {
+            const size_t m = std::min(cap - 1, text.size());
+            std::memcpy(report, text.data(), m);
+            report[m] = 0;
+        }
+    });
+}
+
 int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps) {
     return guarded([&] {
         require(gbps && bytes >= 16 && reps > 0, "invalid copy calibration");

@@ -515,7 +515,9 @@ void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
 
 // Per series length M = N/2: pass-1 length C with S1 rows per block, pass-2 length R
 // with S2 column pairs per block.  Blocks hold 16 elements per thread where the radix
-// plan allows (up to 1024 threads, 16 waves per CU), 48 for the 3 * 2^k plans.
+// plan allows (up to 1024 threads, 16 waves per CU), 48 for the 3 * 2^k plans.  The first
+// row of a length is the default; the others are the autotuner's alternatives (other
+// LDS tiles C x S1 and 2 R x S2, and so other radix sequences).
 struct ShapeRow { uint32_t m, c, r, s1, s2; };
 constexpr ShapeRow kShapes[] = {
     {1u << 12, 64, 64, 64, 32},   {1u << 13, 128, 64, 32, 32},  {1u << 14, 128, 128, 32, 16},
@@ -525,17 +527,34 @@ constexpr ShapeRow kShapes[] = {
     {3u << 14, 384, 128, 32, 16}, {3u << 15, 768, 128, 16, 16}, {3u << 16, 768, 256, 16, 16},
     {3u << 17, 1536, 256, 8, 16}, {3u << 18, 3072, 256, 4, 16}, {3u << 19, 3072, 512, 4, 16},
     {3u << 20, 3072, 1024, 4, 8}, {3u << 21, 3072, 2048, 4, 4},
+    // alternatives
+    {1u << 19, 1024, 512, 16, 16}, {1u << 19, 4096, 128, 4, 16},
+    {1u << 20, 2048, 512, 8, 16}, {1u << 20, 1024, 1024, 16, 8},
+    {1u << 21, 2048, 1024, 8, 8}, {1u << 21, 1024, 2048, 16, 4},
+    {1u << 22, 2048, 2048, 8, 4},
+    {3u << 18, 1536, 512, 8, 16}, {3u << 18, 768, 1024, 16, 8},
+    {3u << 19, 1536, 1024, 8, 8}, {3u << 19, 768, 2048, 16, 4},
+    {3u << 20, 1536, 2048, 8, 4},
 };
 
 }  // namespace
 
 namespace brp_engine {
 
+std::vector<FusedShape> fused_shapes(uint32_t n) {
+    std::vector<FusedShape> out;
+    for (const ShapeRow& row : kShapes) {
+        FusedShape s;
+        if (row.m == n / 2 && fused_supported(n, row.c, &s)) out.push_back(s);
+    }
+    return out;
+}
+
 bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* out) {
     const uint32_t m = n / 2;
     for (const ShapeRow& row : kShapes) {
         if (row.m != m) continue;
-        if (c_override && c_override != row.c) return false;
+        if (c_override && c_override != row.c) continue;
         FusedShape s;
         s.n = n; s.m = m; s.c = row.c; s.r = row.r; s.s1 = row.s1; s.s2 = row.s2;
         s.blocks1 = row.r / row.s1;

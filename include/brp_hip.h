@@ -176,6 +176,16 @@ size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap);
  * number of launch sequences timed: groups of templates (fused engine) or templates
  * (plain engine). */
 int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches);
+/* Autotuner (no reference counterpart; the reference picks its radix fusion and tile by
+ * hand-run sweeps).  For the series length of templates[0], times the fused engine on the
+ * caller's `count` templates (one run of `count` per measurement, `reps` measurements
+ * after a warm-up, median kept) for every compiled four-step split (pass-1 length C and
+ * rows per block, pass-2 length R and columns per block: the LDS tiles and radix
+ * sequences) and every templates-per-launch value in {2, 4, 8, 16} not above count.
+ * *best receives `base` with split_c and group set to the fastest; report (may be NULL)
+ * receives one JSON object per candidate, newline separated. */
+int brp_hip_autotune(const brp_params* templates, size_t count, const float* input, size_t input_count,
+                     const brp_options* base, uint32_t reps, brp_options* best, char* report, size_t cap);
 /* Measurement aid with no reference counterpart: copies `bytes` between two device
  * buffers `reps` times with a plain streaming kernel (brp_copy) and returns the mean
  * GB/s; used to calibrate hardware byte counters against a transfer of known size. */

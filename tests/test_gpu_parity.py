@@ -104,6 +104,47 @@ def test_full_size_2p22(brp, oracle, wl, engine):
     say(rep)
 
 
+# (series length N, pass-1 length C) of every non-default split; N/2 = C x R.
+ALTERNATIVE_SPLITS = [(1 << 20, 1024), (1 << 20, 4096), (1 << 21, 2048), (1 << 21, 1024), (1 << 22, 2048),
+                      (1 << 22, 1024), (1 << 23, 2048), (3 << 19, 1536), (3 << 19, 768), (3 << 20, 1536),
+                      (3 << 20, 768), (3 << 21, 1536)]
+
+
+@pytest.mark.parametrize("n,c", ALTERNATIVE_SPLITS)
+def test_fused_alternative_splits(brp, oracle, wl, n, c):
+    """Every non-default four-step split the autotuner may pick meets the same bar."""
+    w = wl.plain_full(seed=7, n=n, threshold_scale=0.3)
+    p = w.templates[0]
+    say(f"[gpu] fused N={n} split C={c}")
+    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True, split_c=c) as g:
+        g.set_template(p)
+        g.run()
+        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"fused/N={n}/C={c}")
+    say(rep)
+
+
+def test_autotune_picks_a_correct_configuration(brp, wl):
+    """The tuner reports every (split, templates-per-launch) candidate and its pick gives
+    the same candidates as the default configuration."""
+    w = wl.bank(count=8, n=1 << 20)
+    say("[gpu] autotune N=2^20, 8 templates")
+    best, report = brp.autotune(w.templates, w.x, reps=2)
+    say(best, report)
+    assert len(report) == 3 * 3  # splits C = 4096, 2048, 1024 x group 2, 4, 8
+    assert {r["C"] for r in report} == {4096, 2048, 1024}
+    assert best["split_c"] in {4096, 2048, 1024} and best["group"] in {2, 4, 8}
+    assert all(r["us_per_template"] > 0 for r in report)
+    out = []
+    for kw in ({}, best):
+        with brp.GPU(w.templates[0], w.x, 256, 0, 8, packed=True, engine=brp.ENGINE_FUSED, graph=True, **kw) as g:
+            g.set_templates(w.templates)
+            g.run()
+            out.append([g.candidates(s) for s in range(8)])
+    for p, a, b in zip(w.templates, *out):
+        cc = parity.compare_candidates(b, a, p.threshold)
+        assert not cc["mismatches"], cc
+
+
 def test_full_size_mixed_radix(brp, oracle, wl):
     w = wl.mixed_radix(count=3)
     say("[gpu] fused N=3*2^20, 3 templates in one launch")
