@@ -22,6 +22,7 @@ import argparse
 import json
 import os
 import pathlib
+import re
 import sys
 import time
 
@@ -35,6 +36,27 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+KERNEL_PREFIX = {"pass1_resample_fft": "k_pass1<", "pass2_fft_power": "k_pass2<", "harmonic": "k_harmonic<"}
+
+
+def measured_traffic(kernel: str, n: int, shape: dict):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) of
+    `kernel` from the committed rocprofv3 counter summary, when that summary was taken at
+    this series length, pass-1 split and templates per launch; else (None, None)."""
+    path = ROOT / "profiles" / "fused_bank64_n2p22_hbm_traffic.md"
+    try:
+        text = path.read_text()
+        data = json.loads(text[text.index("```json") + 7:text.rindex("```")])
+    except (OSError, ValueError):
+        return None, None
+    if n != 1 << 22 or shape.get("C") != data["split_c"] or shape.get("group") != data["group"]:
+        return None, None
+    for name, v in data["kernels"].items():
+        if name.startswith(KERNEL_PREFIX[kernel]):
+            return v["fetch_bytes"] + v["write_bytes"], f"profiles/{path.name}"
+    return None, None
 
 
 def main() -> int:
@@ -90,7 +112,9 @@ def main() -> int:
     # ---------------------------------------------------------------- batched --
     g = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
                 graph=not args.no_graph, group=args.group, split_c=args.split_c)
-    log("[bench]", g.device_info())
+    info = g.device_info()
+    log("[bench]", info)
+    shape = {k: int(v) for k, v in re.findall(r"\b(C|R|S1|S2|group)=(\d+)", info)}
 
     def bank_pass():
         top = brp.Toplist(100)
@@ -148,6 +172,7 @@ def main() -> int:
     per_launch_ms = kms[dom] / max(launches, 1)
     templates_per_launch = args.templates * args.steps / max(launches, 1)
     achieved = kb[dom] * templates_per_launch / (per_launch_ms * 1e-3) if per_launch_ms > 0 else 0.0
+    traffic, traffic_src = measured_traffic(dom, n, shape)
     pipeline_achieved = B * templates_per_s
 
     # ------------------------------------------------------------ CPU baseline --
@@ -191,7 +216,8 @@ def main() -> int:
         "batched_over_plain": round(templates_per_s / plain_tps, 3) if plain_tps else None,
         "toplist_identical": identical,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "per_launch_ms": round(per_launch_ms, 4), "templates_per_launch": round(templates_per_launch, 3),
                      "algorithmic_bytes_per_template": kb[dom]},
         "pipeline_roofline": {"bytes_per_template": B, "achieved_GBps": round(pipeline_achieved / 1e9, 2),

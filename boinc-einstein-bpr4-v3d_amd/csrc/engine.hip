@@ -559,7 +559,6 @@ int brp_hip_snapshot(brp_hip* h, uint32_t slot, float* resampled, int32_t* index
 }
 
 size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap) {
-    (void)h;
     std::ostringstream s;
     int dev = 0;
     hipDeviceProp_t prop{};
@@ -572,6 +571,11 @@ size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap) {
     } else {
         s << "no HIP device";
     }
+    if (h && h->opt.engine == BRP_ENGINE_FUSED)
+        s << "; engine=fused N=" << h->shape.n << " C=" << h->shape.c << " R=" << h->shape.r << " S1=" << h->shape.s1
+          << " S2=" << h->shape.s2 << " group=" << h->group;
+    else if (h)
+        s << "; engine=plain N=" << h->base.n;
     const std::string text = s.str();
     if (buf && cap) {
         const size_t n = std::min(cap - 1, text.size());
