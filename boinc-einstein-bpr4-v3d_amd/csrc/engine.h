@@ -64,6 +64,8 @@ struct FusedArgs {
     uint32_t first = 0;              // batch slot of group slot 0
     uint32_t count = 0;              // templates in this group
     uint32_t fp64_phase = 0;
+    uint32_t fma = 0;                // flag 2 arithmetic (uniform over a handle)
+    uint32_t nonneg = 0;             // every template of the group has omega, psi, dt >= 0
     // snapshot (per batch slot, or null)
     float* snap_resampled = nullptr;  // N floats
     int32_t* snap_index = nullptr;    // N int32
@@ -72,7 +74,9 @@ struct FusedArgs {
     uint32_t snap_f = 0;
 };
 
-bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* shape);
+// The shape of series length n with pass-1 length c, pass-1 rows per block s1 and pass-2
+// column pairs per block s2 (each 0 = the first match).
+bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShape* shape);
 // Every compiled shape of series length n, the default first.
 std::vector<FusedShape> fused_shapes(uint32_t n);
 // One launch each covers all `count` templates of the group.

@@ -115,6 +115,7 @@ struct brp_hip {
     uint32_t tiles = 0;
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     uint32_t graph_count[2] = {0, 0};
+    std::vector<uint8_t> graph_nonneg[2];  // per group of the captured graph
     bool graph_events = true;
 
     std::vector<hipEvent_t> events[2];
@@ -216,7 +217,7 @@ void create_plain(brp_hip& h) {
 
 void create_fused(brp_hip& h) {
     const brp_params& p = h.base;
-    require(brp_engine::fused_supported(p.n, h.opt.split_c, &h.shape),
+    require(brp_engine::fused_supported(p.n, h.opt.split_c, h.opt.split_s1, h.opt.split_s2, &h.shape),
             "fused engine: unsupported series length (N = 2^13..2^23 or 3*2^15..3*2^22)");
     const auto& s = h.shape;
     h.group = h.opt.snapshot ? h.opt.batch : std::min<uint32_t>(h.opt.batch, h.opt.group ? h.opt.group : 8);
@@ -270,6 +271,20 @@ void enqueue_plain(brp_hip& h, uint32_t r) {
     HIPC(hipEventRecord(h.event(0, 5 * r + 4), st));
 }
 
+// Per group of `half`: whether every template's orbital phase is non-negative for all
+// samples (omega, psi, dt >= 0), which selects the shorter exact index form.
+std::vector<uint8_t> group_nonneg(const brp_hip& h, int half, uint32_t count) {
+    std::vector<uint8_t> out;
+    const auto& tp = h.templates[half];
+    for (uint32_t first = 0; first < count; first += h.group) {
+        bool nn = true;
+        for (uint32_t i = first; i < std::min(count, first + h.group); ++i)
+            nn = nn && tp[i].omega >= 0.0f && tp[i].psi >= 0.0f && tp[i].dt >= 0.0f;
+        out.push_back(nn ? 1 : 0);
+    }
+    return out;
+}
+
 // The fused sequence for `count` templates in slots of `half`: every group of `group`
 // templates in three launches, then the packing of the results for one readback.
 void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
@@ -283,11 +298,14 @@ void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
         if (capturing) HIPC(hipEventRecordWithFlags(h.event(half, i), st, hipEventRecordExternal));
         else HIPC(hipEventRecord(h.event(half, i), st));
     };
+    const std::vector<uint8_t> nonneg = group_nonneg(h, half, count);
     uint32_t gi = 0;
     for (uint32_t first = 0; first < count; first += h.group, ++gi) {
         brp_engine::FusedArgs a = h.fa;
         a.first = uint32_t(base) + first;
         a.count = std::min(h.group, count - first);
+        a.fma = (h.base.flags & 2u) ? 1u : 0u;
+        a.nonneg = nonneg[gi];
         record(4 * gi);
         brp_engine::fused_pass1(st, h.shape, a);
         record(4 * gi + 1);
@@ -304,7 +322,8 @@ void run_fused(brp_hip& h, int half, uint32_t count) {
         enqueue_fused(h, half, count, false);
         return;
     }
-    if (!h.graph_exec[half] || h.graph_count[half] != count) {
+    const std::vector<uint8_t> nonneg = group_nonneg(h, half, count);
+    if (!h.graph_exec[half] || h.graph_count[half] != count || h.graph_nonneg[half] != nonneg) {
         if (h.graph_exec[half]) { HIPC(hipGraphExecDestroy(h.graph_exec[half])); h.graph_exec[half] = nullptr; }
         hipGraph_t graph = nullptr;
         auto capture = [&](bool events) {
@@ -329,6 +348,7 @@ void run_fused(brp_hip& h, int half, uint32_t count) {
         HIPC(hipGraphInstantiate(&h.graph_exec[half], graph, nullptr, nullptr, 0));
         HIPC(hipGraphDestroy(graph));
         h.graph_count[half] = count;
+        h.graph_nonneg[half] = nonneg;
     }
     HIPC(hipGraphLaunch(h.graph_exec[half], h.stream));
 }
@@ -451,8 +471,11 @@ int brp_hip_create(const brp_params* p, const float* input, size_t count, const 
         if (!h->opt.workgroup) h->opt.workgroup = 256;
         if (opt->engine == BRP_ENGINE_FUSED) h->base.flags |= 1u | 64u;  // packed, fused power
         HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        h->x = h->alloc<float>(p->unpadded);
+        // Four zero guard samples after the observation: the fused gather reads three
+        // consecutive samples starting at any index below unpadded.
+        h->x = h->alloc<float>(size_t(p->unpadded) + 4);
         HIPC(hipMemcpy(h->x, input, sizeof(float) * p->unpadded, hipMemcpyHostToDevice));
+        HIPC(hipMemset(h->x + p->unpadded, 0, 4 * sizeof(float)));
         std::vector<float2> lut(65);
         for (int i = 0; i < 65; ++i) lut[i] = make_float2(brp_lut::kTable[2 * i], brp_lut::kTable[2 * i + 1]);
         h->lut = h->upload(lut);
@@ -628,6 +651,8 @@ int brp_hip_autotune(const brp_params* templates, size_t count, const float* inp
                 o.snapshot = 0;
                 o.batch = uint32_t(count);
                 o.split_c = s.c;
+                o.split_s1 = s.s1;
+                o.split_s2 = s.s2;
                 o.group = group;
                 brp_hip* h = nullptr;
                 if (int rc = brp_hip_create(&templates[0], input, input_count, &o, &h)) throw Failure(rc, brp_last_error());
@@ -650,6 +675,8 @@ int brp_hip_autotune(const brp_params* templates, size_t count, const float* inp
                 if (med < best_us) {
                     best_us = med;
                     best->split_c = s.c;
+                    best->split_s1 = s.s1;
+                    best->split_s2 = s.s2;
                     best->group = group;
                 }
             }

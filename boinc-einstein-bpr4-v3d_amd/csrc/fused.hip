@@ -136,34 +136,26 @@ __device__ __forceinline__ cf twiddle2(const float2* __restrict__ lo, const floa
 }
 
 // ------------------------------------------------------------------ pass 1 --
-// SNAP: also write the resampled samples and their source indices (validation runs).
-// F64: orbital offset from the FP64 sine (a reported variant; the FP32 LUT rule is the
-// parity mode).  The hot instance has neither, and its gather is branch-free: every
-// index is computed first, loads use the index clamped into the observation and the
-// value is masked afterwards, so all of a thread's loads can be in flight together.
-template <int C, int S, int T, bool SNAP, bool F64>
-__global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
+// Resample + pack the E elements of thread t: element e (butterfly u = e / R0, input
+// q = e % R0) is the packed pair z[m] = (y[2m], y[2m+1]) with m = r0 + s + R c,
+// c = j + q C/R0.  In chunks of CH elements: every index of the chunk, then every load,
+// then the masking, so all of a thread's loads can be in flight together.  The two
+// samples of a pair come from source indices j0 <= j1 <= j0 + 2 (the orbit moves the
+// index by far less than one sample per sample), so one 12-byte load at
+// b = clamp(j0, 0, U - 1) serves both (the observation buffer holds zero guard samples
+// after U - 1); any other pair takes a second load.  Samples outside the observation
+// are masked to zero afterwards and flagged.
+// NN: phase >= 0 for every sample; FM: flag 2 arithmetic; F64: FP64 orbital phase.
+template <int C, int S, int T, bool SNAP, bool F64, bool NN, bool FM>
+__device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, const FusedArgs& a,
+                                               const FusedShape& sh, const brp_params& p,
+                                               const float4* __restrict__ lut4, uint32_t r0, uint32_t slot,
+                                               int t) {
     constexpr int E = C * S / T;
-    static_assert(plan_ok<C>(E), "pass-1 radix plan does not fit the block");
-    constexpr Radices pl = radices_for(C);
-    constexpr int R0 = pl.r[0], RL = pl.r[pl.count - 1];
-    __shared__ float2 lds[lds_size<C, S>()];
-    __shared__ float2 lut[65];
-    __shared__ double red[T / 64];
-    const int t = threadIdx.x;
-    const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t g = item / sh.blocks1, rb = item % sh.blocks1;
-    const uint32_t slot = a.first + g;
-    const brp_params& p = a.params[slot];
-    const uint32_t R = sh.r, M = sh.m;
-    const uint32_t r0 = rb * S;
+    constexpr int R0 = radices_for(C).r[0];
+    const uint32_t R = sh.r;
     const uint32_t steps = p.steps, U = p.unpadded;
     const Orbit32 o{p.tau, p.omega, p.psi, p.dt, p.step_inv, p.s0, p.flags};
-    if (t < 65) lut[t] = a.lut[t];
-    __syncthreads();
-
-    // Element e of the thread (butterfly u = e / R0, input q = e % R0) is the packed pair
-    // z[m] = (y[2m], y[2m+1]) with m = r0 + s + R c, c = j + q C/R0.
     auto element_m = [&](int e) -> uint32_t {
         const int b = t + T * (e / R0);
         const int s = b % S, j = b / S;
@@ -175,21 +167,12 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
             const double s0_dt = double(p.tau) * sin_f64(double(p.psi)) / double(p.dt);
             return source_index_fp64(i, tau_dt, double(p.omega), double(p.psi), double(p.dt), s0_dt);
         } else {
-            return source_index(i, o, lut);
+            return source_index4<NN, FM>(i, o, lut4);
         }
     };
-
-    // In chunks of CH elements: every index of the chunk, then every load, then the
-    // masking.  The two samples of a pair come from source indices j0 <= j1 <= j0 + 2 (the
-    // orbit moves the index by far less than one sample per sample), so one 12-byte load
-    // at b = clamp(j0, 0, U - 3) serves both; a pair outside its window (only at the ends
-    // of the observation) takes two clamped single loads.
-    constexpr int CH = F64 ? 2 : (E < 8 ? E : 8);
+    constexpr int CH = F64 ? 2 : (E < 4 ? E : 4);  // bounds the live registers of the gather
     static_assert(E % CH == 0, "chunking");
     struct Trio { float v[3]; };
-    double acc = 0.0;
-    bool bad = false;
-    cf v[E];
 #pragma unroll
     for (int c0 = 0; c0 < E; c0 += CH) {
         int src[2 * CH];
@@ -203,17 +186,13 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
 #pragma unroll
         for (int e = 0; e < CH; ++e) {
             const int j0 = src[2 * e], j1 = src[2 * e + 1];
-            const int b = min(max(j0, 0), int(U) - 3);
+            const int b = min(max(j0, 0), int(U) - 1);
             Trio w;
             __builtin_memcpy(&w, a.x + b, sizeof(Trio));
-            const uint32_t d0 = uint32_t(j0 - b), d1 = uint32_t(j1 - b);
-            if (d0 <= 2u && d1 <= 2u) {
-                y[2 * e] = d0 == 0u ? w.v[0] : (d0 == 1u ? w.v[1] : w.v[2]);
-                y[2 * e + 1] = d1 == 0u ? w.v[0] : (d1 == 1u ? w.v[1] : w.v[2]);
-            } else {
-                y[2 * e] = a.x[min(max(j0, 0), int(U) - 1)];
-                y[2 * e + 1] = a.x[min(max(j1, 0), int(U) - 1)];
-            }
+            const uint32_t d1 = uint32_t(j1 - b);
+            y[2 * e] = w.v[0];
+            y[2 * e + 1] = d1 == 0u ? w.v[0] : (d1 == 1u ? w.v[1] : w.v[2]);
+            if (d1 > 2u) y[2 * e + 1] = a.x[min(max(j1, 0), int(U) - 1)];
         }
 #pragma unroll
         for (int e = 0; e < CH; ++e) {
@@ -224,7 +203,7 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
                 const uint32_t i = 2 * m + h;
                 const int jj = src[2 * e + h];
                 const bool in_series = i < steps;
-                const bool inside = jj >= 0 && uint32_t(jj) < U;
+                const bool inside = uint32_t(jj) < U;
                 bad |= in_series && !inside;
                 pair[h] = (in_series && inside) ? y[2 * e + h] : 0.0f;
                 acc += double(pair[h]);
@@ -238,6 +217,38 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
             v[c0 + e] = cf{pair[0], pair[1]};
         }
     }
+}
+
+// SNAP: also write the resampled samples and their source indices (validation runs).
+// F64: orbital offset from the FP64 sine (a reported variant; the FP32 LUT rule is the
+// parity mode).
+// At least 4 waves per SIMD: blocks of 512 threads then run two per CU (128 VGPRs).
+template <int C, int S, int T, bool SNAP, bool F64, bool NN, bool FM>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_pass1(FusedArgs a, FusedShape sh) {
+    constexpr int E = C * S / T;
+    static_assert(plan_ok<C>(E), "pass-1 radix plan does not fit the block");
+    constexpr Radices pl = radices_for(C);
+    constexpr int RL = pl.r[pl.count - 1];
+    __shared__ float2 lds[lds_size<C, S>()];
+    __shared__ float4 lut4[65];
+    __shared__ double red[T / 64];
+    const int t = threadIdx.x;
+    const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t g = item / sh.blocks1, rb = item % sh.blocks1;
+    const uint32_t slot = a.first + g;
+    const brp_params& p = a.params[slot];
+    const uint32_t M = sh.m;
+    const uint32_t r0 = rb * S;
+    if (t < 65) {
+        const float2 sc = a.lut[t];
+        lut4[t] = make_float4(sc.x, sc.y, 0.5f * sc.x, 0.0f);
+    }
+    __syncthreads();
+
+    double acc = 0.0;
+    bool bad = false;
+    cf v[E];
+    resample_pairs<C, S, T, SNAP, F64, NN, FM>(v, acc, bad, a, sh, p, lut4, r0, slot, t);
     fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
 
     float2* Y = a.Y + size_t(g) * M;
@@ -267,7 +278,7 @@ __global__ __launch_bounds__(T) void k_pass1(FusedArgs a, FusedShape sh) {
 // ------------------------------------------------------------------ pass 2 --
 // SNAP: also write the spectrum X (validation runs).
 template <int R, int S, int T, bool SNAP>
-__global__ __launch_bounds__(T) void k_pass2(FusedArgs a, FusedShape sh) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(FusedArgs a, FusedShape sh) {
     constexpr int SS = 2 * S;
     constexpr int E = R * SS / T;
     static_assert(plan_ok<R>(E), "pass-2 radix plan does not fit the block");
@@ -502,15 +513,62 @@ template <int C, int S, int T>
 void launch1(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
     const dim3 grid(sh.blocks1 * a.count), block(T);
     const bool snap = a.snap_resampled != nullptr;
-    if (snap && a.fp64_phase) k_pass1<C, S, T, true, true><<<grid, block, 0, st>>>(a, sh);
-    else if (snap) k_pass1<C, S, T, true, false><<<grid, block, 0, st>>>(a, sh);
-    else if (a.fp64_phase) k_pass1<C, S, T, false, true><<<grid, block, 0, st>>>(a, sh);
-    else k_pass1<C, S, T, false, false><<<grid, block, 0, st>>>(a, sh);
+    // Index form per group (the same in validation runs with snapshots); FP64 phase runs
+    // take the general form.
+    if (a.fp64_phase) {
+        if (snap) k_pass1<C, S, T, true, true, false, false><<<grid, block, 0, st>>>(a, sh);
+        else k_pass1<C, S, T, false, true, false, false><<<grid, block, 0, st>>>(a, sh);
+        return;
+    }
+    auto go = [&]<bool NN, bool FM>() {
+        if (snap) k_pass1<C, S, T, true, false, NN, FM><<<grid, block, 0, st>>>(a, sh);
+        else k_pass1<C, S, T, false, false, NN, FM><<<grid, block, 0, st>>>(a, sh);
+    };
+    if (a.nonneg && a.fma) go.template operator()<true, true>();
+    else if (a.nonneg) go.template operator()<true, false>();
+    else if (a.fma) go.template operator()<false, true>();
+    else go.template operator()<false, false>();
 }
 template <int R, int S, int T>
 void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
     if (a.snap_X) k_pass2<R, S, T, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
     else k_pass2<R, S, T, false><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
+}
+
+// The compiled kernel instances: f.template operator()<C, S1, T>() for pass 1 and
+// <R, S2, T> for pass 2; false when (length, rows) has no instance.
+template <typename F>
+bool with_pass1(uint32_t c, uint32_t s1, F&& f) {
+    switch (c * 1000 + s1) {
+        case 64 * 1000 + 64: f.template operator()<64, 64, 256>(); return true;
+        case 128 * 1000 + 32: f.template operator()<128, 32, 256>(); return true;
+        case 256 * 1000 + 16: f.template operator()<256, 16, 256>(); return true;
+        case 512 * 1000 + 16: f.template operator()<512, 16, 512>(); return true;
+        case 1024 * 1000 + 16: f.template operator()<1024, 16, 1024>(); return true;
+        case 2048 * 1000 + 8: f.template operator()<2048, 8, 1024>(); return true;
+        case 2048 * 1000 + 4: f.template operator()<2048, 4, 512>(); return true;
+        case 4096 * 1000 + 4: f.template operator()<4096, 4, 1024>(); return true;
+        case 4096 * 1000 + 2: f.template operator()<4096, 2, 512>(); return true;
+        case 384 * 1000 + 32: f.template operator()<384, 32, 256>(); return true;
+        case 768 * 1000 + 16: f.template operator()<768, 16, 256>(); return true;
+        case 1536 * 1000 + 8: f.template operator()<1536, 8, 512>(); return true;
+        case 3072 * 1000 + 4: f.template operator()<3072, 4, 256>(); return true;
+        default: return false;
+    }
+}
+template <typename F>
+bool with_pass2(uint32_t r, uint32_t s2, F&& f) {
+    switch (r * 1000 + s2) {
+        case 64 * 1000 + 32: f.template operator()<64, 32, 256>(); return true;
+        case 128 * 1000 + 16: f.template operator()<128, 16, 256>(); return true;
+        case 256 * 1000 + 16: f.template operator()<256, 16, 512>(); return true;
+        case 512 * 1000 + 16: f.template operator()<512, 16, 1024>(); return true;
+        case 512 * 1000 + 8: f.template operator()<512, 8, 512>(); return true;
+        case 1024 * 1000 + 8: f.template operator()<1024, 8, 1024>(); return true;
+        case 1024 * 1000 + 4: f.template operator()<1024, 4, 512>(); return true;
+        case 2048 * 1000 + 4: f.template operator()<2048, 4, 1024>(); return true;
+        default: return false;
+    }
 }
 
 // Per series length M = N/2: pass-1 length C with S1 rows per block, pass-2 length R
@@ -535,6 +593,10 @@ constexpr ShapeRow kShapes[] = {
     {3u << 18, 1536, 512, 8, 16}, {3u << 18, 768, 1024, 16, 8},
     {3u << 19, 1536, 1024, 8, 8}, {3u << 19, 768, 2048, 16, 4},
     {3u << 20, 1536, 2048, 8, 4},
+    // half-size blocks (512 threads, two blocks per CU)
+    {1u << 19, 2048, 256, 4, 16}, {1u << 20, 4096, 256, 2, 16},
+    {1u << 21, 4096, 512, 2, 8}, {1u << 21, 4096, 512, 2, 16}, {1u << 21, 4096, 512, 4, 8},
+    {1u << 21, 2048, 1024, 4, 4}, {1u << 22, 4096, 1024, 2, 4},
 };
 
 }  // namespace
@@ -545,16 +607,18 @@ std::vector<FusedShape> fused_shapes(uint32_t n) {
     std::vector<FusedShape> out;
     for (const ShapeRow& row : kShapes) {
         FusedShape s;
-        if (row.m == n / 2 && fused_supported(n, row.c, &s)) out.push_back(s);
+        if (row.m == n / 2 && fused_supported(n, row.c, row.s1, row.s2, &s)) out.push_back(s);
     }
     return out;
 }
 
-bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* out) {
+bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShape* out) {
     const uint32_t m = n / 2;
     for (const ShapeRow& row : kShapes) {
         if (row.m != m) continue;
-        if (c_override && c_override != row.c) continue;
+        if ((c && c != row.c) || (s1 && s1 != row.s1) || (s2 && s2 != row.s2)) continue;
+        if (!with_pass1(row.c, row.s1, []<int, int, int>() {}) || !with_pass2(row.r, row.s2, []<int, int, int>() {}))
+            continue;
         FusedShape s;
         s.n = n; s.m = m; s.c = row.c; s.r = row.r; s.s1 = row.s1; s.s2 = row.s2;
         s.blocks1 = row.r / row.s1;
@@ -566,32 +630,11 @@ bool fused_supported(uint32_t n, uint32_t c_override, FusedShape* out) {
 }
 
 void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
-    switch (s.c * 1000 + s.s1) {
-        case 64 * 1000 + 64: launch1<64, 64, 256>(st, s, a); break;
-        case 128 * 1000 + 32: launch1<128, 32, 256>(st, s, a); break;
-        case 256 * 1000 + 16: launch1<256, 16, 256>(st, s, a); break;
-        case 512 * 1000 + 16: launch1<512, 16, 512>(st, s, a); break;
-        case 1024 * 1000 + 16: launch1<1024, 16, 1024>(st, s, a); break;
-        case 2048 * 1000 + 8: launch1<2048, 8, 1024>(st, s, a); break;
-        case 4096 * 1000 + 4: launch1<4096, 4, 1024>(st, s, a); break;
-        case 384 * 1000 + 32: launch1<384, 32, 256>(st, s, a); break;
-        case 768 * 1000 + 16: launch1<768, 16, 256>(st, s, a); break;
-        case 1536 * 1000 + 8: launch1<1536, 8, 512>(st, s, a); break;
-        case 3072 * 1000 + 4: launch1<3072, 4, 256>(st, s, a); break;
-        default: break;
-    }
+    with_pass1(s.c, s.s1, [&]<int C, int S, int T>() { launch1<C, S, T>(st, s, a); });
 }
 
 void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
-    switch (s.r * 1000 + s.s2) {
-        case 64 * 1000 + 32: launch2<64, 32, 256>(st, s, a); break;
-        case 128 * 1000 + 16: launch2<128, 16, 256>(st, s, a); break;
-        case 256 * 1000 + 16: launch2<256, 16, 512>(st, s, a); break;
-        case 512 * 1000 + 16: launch2<512, 16, 1024>(st, s, a); break;
-        case 1024 * 1000 + 8: launch2<1024, 8, 1024>(st, s, a); break;
-        case 2048 * 1000 + 4: launch2<2048, 4, 1024>(st, s, a); break;
-        default: break;
-    }
+    with_pass2(s.r, s.s2, [&]<int R, int S, int T>() { launch2<R, S, T>(st, s, a); });
 }
 
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles) {

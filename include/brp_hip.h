@@ -80,7 +80,9 @@ typedef struct brp_options {
     uint32_t split_c;      /* fused engine: pass-1 FFT length C (0 = default table) */
     uint32_t precision;    /* 0 = FP32 FFT; 1 = split-FP16 MFMA first radix-16 stages */
     uint32_t group;        /* fused engine: templates per launch (0 = 8) */
-    uint32_t reserved[5];
+    uint32_t split_s1;     /* fused engine: pass-1 rows per block (0 = default for split_c) */
+    uint32_t split_s2;     /* fused engine: pass-2 column pairs per block (0 = default) */
+    uint32_t reserved[3];
 } brp_options;
 
 /* src/vulkan.hpp:6 Timings; template_stage_ms is returned separately. */
@@ -180,9 +182,11 @@ int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches);
  * hand-run sweeps).  For the series length of templates[0], times the fused engine on the
  * caller's `count` templates (one run of `count` per measurement, `reps` measurements
  * after a warm-up, median kept) for every compiled four-step split (pass-1 length C and
- * rows per block, pass-2 length R and columns per block: the LDS tiles and radix
- * sequences) and every templates-per-launch value in {2, 4, 8, 16} not above count.
- * *best receives `base` with split_c and group set to the fastest; report (may be NULL)
+ * rows per block S1, pass-2 length R and column pairs per block S2: the LDS tiles, block
+ * sizes and radix sequences) and every templates-per-launch value in {2, 4, 8, 16} not
+ * above count.
+ * *best receives `base` with split_c, split_s1, split_s2 and group set to the fastest; report
+ * (may be NULL)
  * receives one JSON object per candidate, newline separated. */
 int brp_hip_autotune(const brp_params* templates, size_t count, const float* input, size_t input_count,
                      const brp_options* base, uint32_t reps, brp_options* best, char* report, size_t cap);

@@ -62,6 +62,30 @@ def test_fused_small(brp, oracle, wl, scale, flags):
     say(rep)
 
 
+@pytest.mark.parametrize("flags", [0, 6])
+def test_fused_negative_phase(brp, oracle, wl, flags):
+    """psi < 0: phases below zero take the general index form (wrap of a negative
+    fraction), the other templates of the bank the non-negative form."""
+    w = wl.plain_small(threshold_scale=0.3, flags=flags)
+    p = w.templates[0].copy()
+    p.psi = -1.3
+    brp.resampling_steps(p)
+    say(f"[gpu] fused N=65536 psi=-1.3 flags={flags}")
+    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True) as g:
+        g.set_template(p)
+        g.run()
+        snap = g.snapshot(0)
+        rep = parity.check_template(snap, oracle, w.x, p, "fused/psi<0")
+    say(rep)
+    assert rep["candidates"]["ref"] > 0
+    # The same template through the hot (no snapshot, graph) instance: same candidates.
+    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, graph=True) as g:
+        g.set_template(p)
+        g.run()
+        hot = g.candidates(0)
+    assert np.array_equal(hot, snap.candidates)
+
+
 @pytest.mark.parametrize("n", [3 << 15, 1 << 17, 3 << 17])
 def test_fused_lengths(brp, oracle, wl, n):
     w = wl.plain_full(seed=5, n=n, threshold_scale=0.3)
@@ -104,22 +128,30 @@ def test_full_size_2p22(brp, oracle, wl, engine):
     say(rep)
 
 
-# (series length N, pass-1 length C) of every non-default split; N/2 = C x R.
-ALTERNATIVE_SPLITS = [(1 << 20, 1024), (1 << 20, 4096), (1 << 21, 2048), (1 << 21, 1024), (1 << 22, 2048),
-                      (1 << 22, 1024), (1 << 23, 2048), (3 << 19, 1536), (3 << 19, 768), (3 << 20, 1536),
-                      (3 << 20, 768), (3 << 21, 1536)]
+# (series length N, pass-1 length C, rows per pass-1 block S1, column pairs per pass-2
+# block S2) of every non-default shape; N/2 = C x R.
+ALTERNATIVE_SPLITS = [
+    (1 << 20, 1024, 16, 16), (1 << 20, 4096, 4, 16), (1 << 20, 2048, 4, 16),
+    (1 << 21, 2048, 8, 16), (1 << 21, 1024, 16, 8), (1 << 21, 4096, 2, 16),
+    (1 << 22, 2048, 8, 8), (1 << 22, 1024, 16, 4), (1 << 22, 4096, 2, 8), (1 << 22, 4096, 2, 16),
+    (1 << 22, 4096, 4, 8), (1 << 22, 2048, 4, 4),
+    (1 << 23, 2048, 8, 4), (1 << 23, 4096, 2, 4),
+    (3 << 19, 1536, 8, 16), (3 << 19, 768, 16, 8), (3 << 20, 1536, 8, 8), (3 << 20, 768, 16, 4),
+    (3 << 21, 1536, 8, 4)]
 
 
-@pytest.mark.parametrize("n,c", ALTERNATIVE_SPLITS)
-def test_fused_alternative_splits(brp, oracle, wl, n, c):
-    """Every non-default four-step split the autotuner may pick meets the same bar."""
+@pytest.mark.parametrize("n,c,s1,s2", ALTERNATIVE_SPLITS)
+def test_fused_alternative_splits(brp, oracle, wl, n, c, s1, s2):
+    """Every non-default four-step shape the autotuner may pick meets the same bar."""
     w = wl.plain_full(seed=7, n=n, threshold_scale=0.3)
     p = w.templates[0]
-    say(f"[gpu] fused N={n} split C={c}")
-    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True, split_c=c) as g:
+    say(f"[gpu] fused N={n} split C={c} S1={s1} S2={s2}")
+    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True, split_c=c,
+                 split_s1=s1, split_s2=s2) as g:
+        assert f"C={c} " in g.device_info() and f"S1={s1} S2={s2}" in g.device_info()
         g.set_template(p)
         g.run()
-        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"fused/N={n}/C={c}")
+        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"fused/N={n}/C={c}/S1={s1}/S2={s2}")
     say(rep)
 
 
@@ -130,9 +162,10 @@ def test_autotune_picks_a_correct_configuration(brp, wl):
     say("[gpu] autotune N=2^20, 8 templates")
     best, report = brp.autotune(w.templates, w.x, reps=2)
     say(best, report)
-    assert len(report) == 3 * 3  # splits C = 4096, 2048, 1024 x group 2, 4, 8
-    assert {r["C"] for r in report} == {4096, 2048, 1024}
-    assert best["split_c"] in {4096, 2048, 1024} and best["group"] in {2, 4, 8}
+    shapes = {(r["C"], r["S1"], r["S2"]) for r in report}
+    assert len(report) == 3 * len(shapes) and len(shapes) >= 3  # every shape x group 2, 4, 8
+    assert {c for c, _, _ in shapes} == {4096, 2048, 1024}
+    assert (best["split_c"], best["split_s1"], best["split_s2"]) in shapes and best["group"] in {2, 4, 8}
     assert all(r["us_per_template"] > 0 for r in report)
     out = []
     for kw in ({}, best):
