@@ -69,6 +69,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=64, help="templates per run (toplist thresholds frozen per run)")
     ap.add_argument("--group", type=int, default=8, help="templates per kernel launch (without --no-tune: tuned)")
     ap.add_argument("--split-c", type=int, default=0, help="pass-1 FFT length (without --no-tune: tuned)")
+    ap.add_argument("--split-s1", type=int, default=0, help="rows per pass-1 block (without --no-tune: tuned)")
+    ap.add_argument("--split-s2", type=int, default=0, help="column pairs per pass-2 block (without --no-tune: tuned)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner, use --group / --split-c")
     ap.add_argument("--tune-templates", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
@@ -105,13 +107,15 @@ def main() -> int:
         t2 = time.time()
         best, report = brp.autotune(w.templates[:args.tune_templates], w.x,
                                     {"graph": int(not args.no_graph)}, reps=3)
-        args.group, args.split_c = best["group"], best["split_c"]
+        args.group, args.split_c, args.split_s1, args.split_s2 = (best["group"], best["split_c"], best["split_s1"],
+                                                                   best["split_s2"])
         tuned = {"picked": best, "candidates": report, "seconds": round(time.time() - t2, 2)}
         log(f"[bench] autotune picked {best} in {tuned['seconds']} s: {report}")
 
     # ---------------------------------------------------------------- batched --
     g = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
-                graph=not args.no_graph, group=args.group, split_c=args.split_c)
+                graph=not args.no_graph, group=args.group, split_c=args.split_c, split_s1=args.split_s1,
+                split_s2=args.split_s2)
     info = g.device_info()
     log("[bench]", info)
     shape = {k: int(v) for k, v in re.findall(r"\b(C|R|S1|S2|group)=(\d+)", info)}
@@ -210,7 +214,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"bank of {args.templates} orbital templates x N={n} series (bank semantics)",
                    "templates": args.templates, "n": n, "batch": args.batch, "group": args.group,
-                   "split_c": args.split_c, "graph": not args.no_graph},
+                   "split_c": shape.get("C"), "split_s1": shape.get("S1"), "split_s2": shape.get("S2"),
+                   "graph": not args.no_graph},
         "autotune": tuned,
         "plain_templates_per_s": round(plain_tps, 3) if plain_tps else None,
         "batched_over_plain": round(templates_per_s / plain_tps, 3) if plain_tps else None,

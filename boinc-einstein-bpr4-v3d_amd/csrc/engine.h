@@ -53,6 +53,7 @@ struct FusedArgs {
     const float2* tw_r = nullptr;    // W_R^t, t < R
     const float2* twm_lo = nullptr;  // W_M^t, t < 2048
     const float2* twm_hi = nullptr;  // W_M^{2048 q}
+    const float2* tw_rq = nullptr;   // W_{16 R}^t, t < 16 R
     const float2* twn_lo = nullptr;  // W_N^t, t < 2048
     const float2* twn_hi = nullptr;  // W_N^{2048 q}
     float2* Y = nullptr;             // per group slot: M complex

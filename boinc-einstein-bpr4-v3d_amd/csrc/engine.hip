@@ -228,6 +228,7 @@ void create_fused(brp_hip& h) {
     a.tw_r = h.upload(twiddles(s.r, s.r));
     a.twm_lo = h.upload(twiddles(s.m, 2048));
     a.twm_hi = h.upload(twiddles(s.m, (s.m + 2047) / 2048, 2048));
+    a.tw_rq = h.upload(twiddles(16ull * s.r, 16ull * s.r));
     a.twn_lo = h.upload(twiddles(s.n, 2048));
     a.twn_hi = h.upload(twiddles(s.n, (s.n + 2047) / 2048, 2048));
     a.Y = h.alloc<float2>(size_t(h.group) * s.m);

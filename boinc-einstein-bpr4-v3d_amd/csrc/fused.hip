@@ -170,33 +170,46 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
             return source_index4<NN, FM>(i, o, lut4);
         }
     };
-    constexpr int CH = F64 ? 2 : (E < 4 ? E : 4);  // bounds the live registers of the gather
+    constexpr int CH = 2;  // elements per chunk: bounds the live registers of the gather
+    constexpr int K = E / CH;
     static_assert(E % CH == 0, "chunking");
     struct Trio { float v[3]; };
-#pragma unroll
-    for (int c0 = 0; c0 < E; c0 += CH) {
-        int src[2 * CH];
+    auto indices = [&](int k, int* src) {
 #pragma unroll
         for (int e = 0; e < CH; ++e) {
-            const uint32_t m = element_m(c0 + e);
+            const uint32_t m = element_m(k * CH + e);
             src[2 * e] = index_of(2 * m);
             src[2 * e + 1] = index_of(2 * m + 1);
         }
-        float y[2 * CH];
+    };
+    auto issue = [&](const int* src, Trio* w) {
 #pragma unroll
         for (int e = 0; e < CH; ++e) {
-            const int j0 = src[2 * e], j1 = src[2 * e + 1];
-            const int b = min(max(j0, 0), int(U) - 1);
-            Trio w;
-            __builtin_memcpy(&w, a.x + b, sizeof(Trio));
-            const uint32_t d1 = uint32_t(j1 - b);
-            y[2 * e] = w.v[0];
-            y[2 * e + 1] = d1 == 0u ? w.v[0] : (d1 == 1u ? w.v[1] : w.v[2]);
-            if (d1 > 2u) y[2 * e + 1] = a.x[min(max(j1, 0), int(U) - 1)];
+            const int b = min(max(src[2 * e], 0), int(U) - 1);
+            __builtin_memcpy(&w[e], a.x + b, sizeof(Trio));
+        }
+    };
+    auto consume = [&](int k, const int* src, const Trio* w) {
+        float y[2 * CH];
+        bool odd = false;
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            const int b = min(max(src[2 * e], 0), int(U) - 1);
+            const uint32_t d1 = uint32_t(src[2 * e + 1] - b);
+            y[2 * e] = w[e].v[0];
+            y[2 * e + 1] = d1 == 0u ? w[e].v[0] : (d1 == 1u ? w[e].v[1] : w[e].v[2]);
+            odd |= d1 > 2u;
+        }
+        if (odd) {  // a pair whose second index is not within two samples of the first
+#pragma unroll
+            for (int e = 0; e < CH; ++e) {
+                const int b = min(max(src[2 * e], 0), int(U) - 1);
+                if (uint32_t(src[2 * e + 1] - b) > 2u) y[2 * e + 1] = a.x[min(max(src[2 * e + 1], 0), int(U) - 1)];
+            }
         }
 #pragma unroll
         for (int e = 0; e < CH; ++e) {
-            const uint32_t m = element_m(c0 + e);
+            const uint32_t m = element_m(k * CH + e);
             float pair[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -214,8 +227,22 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
                     }
                 }
             }
-            v[c0 + e] = cf{pair[0], pair[1]};
+            v[k * CH + e] = cf{pair[0], pair[1]};
         }
+    };
+    // Two chunks in flight: the indices and loads of chunk k + 1 are issued before the
+    // samples of chunk k are consumed.
+    int src[2][2 * CH];
+    Trio w[2][CH];
+    indices(0, src[0]);
+    issue(src[0], w[0]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (k + 1 < K) {
+            indices(k + 1, src[(k + 1) & 1]);
+            issue(src[(k + 1) & 1], w[(k + 1) & 1]);
+        }
+        consume(k, src[k & 1], w[k & 1]);
     }
 }
 
@@ -251,16 +278,24 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     resample_pairs<C, S, T, SNAP, F64, NN, FM>(v, acc, bad, a, sh, p, lut4, r0, slot, t);
     fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
 
+    // Y[r][k2] = v * W_M^{r k2} for k2 = j + q C/RL: W_M^{r j} times
+    // W_M^{r q C/RL} = W_{R RL}^{r q} (r q < R RL, read from the W_{16 R} table), all of a
+    // thread's table reads issued together.
     float2* Y = a.Y + size_t(g) * M;
 #pragma unroll
     for (int u = 0; u < E / RL; ++u) {
         const int b = t + T * u;
         const int s = b % S, j = b / S;
         const uint32_t r = r0 + s;
+        cf pw[RL];
+#pragma unroll
+        for (int q = 1; q < RL; ++q) pw[q] = load(a.tw_rq + __umul24(r, uint32_t(q * (16 / RL))));
+        const cf base = twiddle2(a.twm_lo, a.twm_hi, __umul24(r, uint32_t(j)));
+        float2* row = Y + size_t(r) * C + j;
 #pragma unroll
         for (int q = 0; q < RL; ++q) {
-            const uint32_t k2 = uint32_t(j + q * (C / RL));
-            store(Y + size_t(r) * C + k2, mul(v[u * RL + q], twiddle2(a.twm_lo, a.twm_hi, __umul24(r, k2))));
+            const cf w = q == 0 ? base : mul(base, pw[q]);
+            store(row + q * (C / RL), mul(v[u * RL + q], w));
         }
     }
 
@@ -295,9 +330,30 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     const bool middle = w == C / (2 * S);  // the block that owns the self-paired column C/2
     const uint32_t base = middle ? C / 2 : w * S;
 
-    // Pad mean from the pass-1 partial sums, in a fixed order (bit-reproducible).
-    double part = 0.0;
-    for (uint32_t b = t; b < sh.blocks1; b += T) part += a.partial[size_t(g) * sh.blocks1 + b];
+    auto column = [&](int s) -> uint32_t {
+        const uint32_t lower = base + uint32_t(s % S);
+        return s < S ? lower : (C - lower) % C;
+    };
+
+    // This block's share of the pass-1 partial sums, read before the columns so that the
+    // mean reduction below waits only for it.
+    const double* partial = a.partial + size_t(g) * sh.blocks1;
+    double part = uint32_t(t) < sh.blocks1 ? partial[t] : 0.0;
+
+    const float2* Y = a.Y + size_t(g) * M;
+    cf v[E];
+#pragma unroll
+    for (int u = 0; u < E / R0; ++u) {
+        const int b = t + T * u;
+        const int s = b % SS, j = b / SS;
+        const uint32_t col = column(s);
+#pragma unroll
+        for (int q = 0; q < R0; ++q) v[u * R0 + q] = load(Y + size_t(j + q * (R / R0)) * C + col);
+    }
+
+    // Pad mean from the pass-1 partial sums, in a fixed order (bit-reproducible), while
+    // the column loads are in flight.
+    for (uint32_t b = t + T; b < sh.blocks1; b += T) part += partial[b];
     for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
     if ((t & 63) == 0) red[t >> 6] = part;
     __syncthreads();
@@ -310,21 +366,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     const float mu = float(red[T / 64] / double(p.steps));
     if (w == 0 && t == 0) a.mean[slot] = mu;
 
-    auto column = [&](int s) -> uint32_t {
-        const uint32_t lower = base + uint32_t(s % S);
-        return s < S ? lower : (C - lower) % C;
-    };
-
-    const float2* Y = a.Y + size_t(g) * M;
-    cf v[E];
-#pragma unroll
-    for (int u = 0; u < E / R0; ++u) {
-        const int b = t + T * u;
-        const int s = b % SS, j = b / SS;
-        const uint32_t col = column(s);
-#pragma unroll
-        for (int q = 0; q < R0; ++q) v[u * R0 + q] = load(Y + size_t(j + q * (R / R0)) * C + col);
-    }
     fft_stages<R, SS, T, 0>(v, lds, a.tw_r, t);
     __syncthreads();
 #pragma unroll
@@ -354,8 +395,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     const uint32_t k_first = uint32_t(t / S) * C + base + uint32_t(t % S);
     cf we = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(k_first) * steps_n));
     const cf wstep = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(T / S) * C % N * steps_n));
+    // Every W_N^k of the thread's pairs first, so their table reads are in flight together.
+    constexpr int PAIRS = R * S / T;
+    cf Wk[PAIRS];
 #pragma unroll
-    for (int u = 0; u < R * S / T; ++u) {
+    for (int u = 0; u < PAIRS; ++u) {
+        const int pi = t + T * u;
+        Wk[u] = twiddle2(a.twn_lo, a.twn_hi, uint32_t(pi / S) * C + base + uint32_t(pi % S));
+    }
+#pragma unroll
+    for (int u = 0; u < PAIRS; ++u) {
         const int pi = t + T * u;
         const int s = pi % S;
         const uint32_t k1 = uint32_t(pi / S);
@@ -367,7 +416,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
         const uint32_t k = k1 * C + col;
         const cf zk = load(lds + lds_at<SS>(int(k1), s));
         const cf zm = load(lds + lds_at<SS>(int(k1m), s + S));
-        const cf W = twiddle2(a.twn_lo, a.twn_hi, k);
+        const cf W = Wk[u];
         const cf Wm = cf{-W.x, W.y};  // W_N^{M-k} = -conj(W_N^k)
         cf xk = scale(add(add(zk, conj(zm)), mul_negi(mul(sub(zk, conj(zm)), W))), 0.5f);
         cf xm = scale(add(add(zm, conj(zk)), mul_negi(mul(sub(zm, conj(zk)), Wm))), 0.5f);
