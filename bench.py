@@ -71,6 +71,7 @@ def main() -> int:
     ap.add_argument("--split-c", type=int, default=0, help="pass-1 FFT length (without --no-tune: tuned)")
     ap.add_argument("--split-s1", type=int, default=0, help="rows per pass-1 block (without --no-tune: tuned)")
     ap.add_argument("--split-s2", type=int, default=0, help="column pairs per pass-2 block (without --no-tune: tuned)")
+    ap.add_argument("--lanes", type=int, default=1, help="groups in flight (without --no-tune: tuned)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner, use --group / --split-c")
     ap.add_argument("--tune-templates", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
@@ -105,17 +106,18 @@ def main() -> int:
     tuned = None
     if not args.no_tune:
         t2 = time.time()
+        # Lanes stay as given: with two lanes the per-kernel event times overlap.
         best, report = brp.autotune(w.templates[:args.tune_templates], w.x,
-                                    {"graph": int(not args.no_graph)}, reps=3)
-        args.group, args.split_c, args.split_s1, args.split_s2 = (best["group"], best["split_c"], best["split_s1"],
-                                                                   best["split_s2"])
+                                    {"graph": int(not args.no_graph), "lanes": args.lanes}, reps=3)
+        args.group, args.split_c, args.split_s1, args.split_s2, args.lanes = (
+            best["group"], best["split_c"], best["split_s1"], best["split_s2"], best["lanes"])
         tuned = {"picked": best, "candidates": report, "seconds": round(time.time() - t2, 2)}
         log(f"[bench] autotune picked {best} in {tuned['seconds']} s: {report}")
 
     # ---------------------------------------------------------------- batched --
     g = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
                 graph=not args.no_graph, group=args.group, split_c=args.split_c, split_s1=args.split_s1,
-                split_s2=args.split_s2)
+                split_s2=args.split_s2, lanes=args.lanes)
     info = g.device_info()
     log("[bench]", info)
     shape = {k: int(v) for k, v in re.findall(r"\b(C|R|S1|S2|group)=(\d+)", info)}
@@ -145,6 +147,21 @@ def main() -> int:
     g.close()
     log(f"[bench] batched: {templates_per_s:.1f} templates/s; kernel ms over the timed region {km.tolist()} "
         f"in {launches} launches")
+
+    # The same configuration without per-kernel events in the graphs.
+    gu = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
+                 graph=not args.no_graph, group=args.group, split_c=args.split_c, split_s1=args.split_s1,
+                 split_s2=args.split_s2, lanes=args.lanes, untimed=True)
+    gu.bank_search(w.orbits, U, workloads.SAMPLE_US, brp.Toplist(100))
+    tu0 = time.perf_counter()
+    for i in range(args.steps):
+        top_u = brp.Toplist(100)
+        gu.bank_search(w.orbits, U, workloads.SAMPLE_US, top_u)
+    untimed_tps = args.templates * args.steps / (time.perf_counter() - tu0)
+    untimed_identical = [(r.bin, r.harmonics, r.power) for r in top_u.table()] == \
+        [(r[0], r[1], r[3]) for r in batched_rows]
+    gu.close()
+    log(f"[bench] batched without kernel events: {untimed_tps:.1f} templates/s, toplist identical {untimed_identical}")
 
     # ------------------------------------------------------------------ plain --
     plain_n = min(args.plain_templates, args.templates)
@@ -215,8 +232,11 @@ def main() -> int:
         "config": {"workload": f"bank of {args.templates} orbital templates x N={n} series (bank semantics)",
                    "templates": args.templates, "n": n, "batch": args.batch, "group": args.group,
                    "split_c": shape.get("C"), "split_s1": shape.get("S1"), "split_s2": shape.get("S2"),
+                   "lanes": args.lanes,
                    "graph": not args.no_graph},
         "autotune": tuned,
+        "untimed_templates_per_s": round(untimed_tps, 3),
+        "untimed_toplist_identical": untimed_identical,
         "plain_templates_per_s": round(plain_tps, 3) if plain_tps else None,
         "batched_over_plain": round(templates_per_s / plain_tps, 3) if plain_tps else None,
         "toplist_identical": identical,

@@ -113,6 +113,11 @@ struct brp_hip {
     brp_engine::FusedArgs fa;
     uint32_t group = 1;
     uint32_t tiles = 0;
+    uint32_t lanes = 1;              // groups in flight, on stream and side
+    bool timed = true;               // per-kernel events
+    hipStream_t side = nullptr;      // second lane
+    hipEvent_t fork = nullptr, join = nullptr;
+    size_t lane_y = 0, lane_p = 0, lane_partial = 0;  // per-lane buffer strides (elements)
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     uint32_t graph_count[2] = {0, 0};
     std::vector<uint8_t> graph_nonneg[2];  // per group of the captured graph
@@ -160,6 +165,9 @@ struct brp_hip {
         for (void* p : allocations) hipFree(p);
         if (staging_host) hipHostFree(staging_host);
         if (params_host) hipHostFree(params_host);
+        if (fork) hipEventDestroy(fork);
+        if (join) hipEventDestroy(join);
+        if (side) hipStreamDestroy(side);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -231,9 +239,20 @@ void create_fused(brp_hip& h) {
     a.tw_rq = h.upload(twiddles(16ull * s.r, 16ull * s.r));
     a.twn_lo = h.upload(twiddles(s.n, 2048));
     a.twn_hi = h.upload(twiddles(s.n, (s.n + 2047) / 2048, 2048));
-    a.Y = h.alloc<float2>(size_t(h.group) * s.m);
-    a.P = h.alloc<float>(size_t(h.group) * s.m);
-    a.partial = h.alloc<double>(size_t(h.group) * s.blocks1);
+    h.lanes = h.opt.snapshot ? 1 : (h.opt.lanes ? h.opt.lanes : 1);
+    require(h.lanes <= 2, "fused engine: lanes must be 1 or 2");
+    h.timed = !h.opt.untimed;
+    h.lane_y = size_t(h.group) * s.m;
+    h.lane_p = size_t(h.group) * s.m;
+    h.lane_partial = size_t(h.group) * s.blocks1;
+    a.Y = h.alloc<float2>(h.lanes * h.lane_y);
+    a.P = h.alloc<float>(h.lanes * h.lane_p);
+    a.partial = h.alloc<double>(h.lanes * h.lane_partial);
+    if (h.lanes == 2) {
+        HIPC(hipStreamCreateWithFlags(&h.side, hipStreamNonBlocking));
+        HIPC(hipEventCreateWithFlags(&h.fork, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&h.join, hipEventDisableTiming));
+    }
     a.mean = h.alloc<float>(2 * size_t(h.opt.batch));
     a.results = h.results;
     a.slot_words = h.slot_words;
@@ -287,13 +306,20 @@ std::vector<uint8_t> group_nonneg(const brp_hip& h, int half, uint32_t count) {
 }
 
 // The fused sequence for `count` templates in slots of `half`: every group of `group`
-// templates in three launches, then the packing of the results for one readback.
+// templates in three launches, then the packing of the results for one readback.  With
+// two lanes, even groups run on the handle's stream and odd groups on the side stream,
+// each lane with its own exchange / power / partial buffers, so one group's launches
+// overlap the other's (and the gaps between them).
 void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
-    hipStream_t st = h.stream;
     const size_t base = h.slot_base(half);
     uint32_t* results = h.results + base * h.slot_words;
-    brp_engine::plain_clear_slots(st, results, h.slot_words, count);
-    const bool ev = !capturing || h.graph_events;
+    brp_engine::plain_clear_slots(h.stream, results, h.slot_words, count);
+    const bool ev = h.timed && (!capturing || h.graph_events);
+    if (h.lanes == 2) {
+        HIPC(hipEventRecord(h.fork, h.stream));
+        HIPC(hipStreamWaitEvent(h.side, h.fork, 0));
+    }
+    hipStream_t st = h.stream;
     auto record = [&](size_t i) {
         if (!ev) return;
         if (capturing) HIPC(hipEventRecordWithFlags(h.event(half, i), st, hipEventRecordExternal));
@@ -303,6 +329,11 @@ void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
     uint32_t gi = 0;
     for (uint32_t first = 0; first < count; first += h.group, ++gi) {
         brp_engine::FusedArgs a = h.fa;
+        const uint32_t lane = gi % h.lanes;
+        st = lane ? h.side : h.stream;
+        a.Y += lane * h.lane_y;
+        a.P += lane * h.lane_p;
+        a.partial += lane * h.lane_partial;
         a.first = uint32_t(base) + first;
         a.count = std::min(h.group, count - first);
         a.fma = (h.base.flags & 2u) ? 1u : 0u;
@@ -315,6 +346,11 @@ void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
         brp_engine::fused_harmonic(st, h.shape, a, h.tiles);
         record(4 * gi + 3);
     }
+    if (h.lanes == 2) {
+        HIPC(hipEventRecord(h.join, h.side));
+        HIPC(hipStreamWaitEvent(h.stream, h.join, 0));
+    }
+    st = h.stream;
     brp_engine::pack_results(st, results, h.slot_words, count, h.keep, h.staging_dev + base * (h.keep + 1) * 4);
 }
 
@@ -369,7 +405,7 @@ void fill_timings(brp_hip& h, int half, uint32_t count) {
             }
         return;
     }
-    if (h.opt.graph && !h.opt.snapshot && !h.graph_events) return;
+    if (!h.timed || (h.opt.graph && !h.opt.snapshot && !h.graph_events)) return;
     const uint32_t groups = (count + h.group - 1) / h.group;
     for (uint32_t gi = 0; gi < groups; ++gi) {
         const uint32_t first = gi * h.group, n = std::min(h.group, count - first);
@@ -597,7 +633,7 @@ size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap) {
     }
     if (h && h->opt.engine == BRP_ENGINE_FUSED)
         s << "; engine=fused N=" << h->shape.n << " C=" << h->shape.c << " R=" << h->shape.r << " S1=" << h->shape.s1
-          << " S2=" << h->shape.s2 << " group=" << h->group;
+          << " S2=" << h->shape.s2 << " group=" << h->group << " lanes=" << h->lanes;
     else if (h)
         s << "; engine=plain N=" << h->base.n;
     const std::string text = s.str();
@@ -644,43 +680,48 @@ int brp_hip_autotune(const brp_params* templates, size_t count, const float* inp
         double best_us = 1e300;
         *best = *base;
         for (const auto& s : shapes) {
-            for (uint32_t group : {2u, 4u, 8u, 16u}) {
-                if (group > count) continue;
-                brp_options o = *base;
-                o.engine = BRP_ENGINE_FUSED;
-                o.packed = 1;
-                o.snapshot = 0;
-                o.batch = uint32_t(count);
-                o.split_c = s.c;
-                o.split_s1 = s.s1;
-                o.split_s2 = s.s2;
-                o.group = group;
-                brp_hip* h = nullptr;
-                if (int rc = brp_hip_create(&templates[0], input, input_count, &o, &h)) throw Failure(rc, brp_last_error());
-                std::unique_ptr<brp_hip> owner(h);
-                if (int rc = brp_hip_set_templates(h, templates, count)) throw Failure(rc, brp_last_error());
-                launch(*h, 0);
-                finish(*h, 0);  // warm-up (and graph capture)
-                std::vector<double> us;
-                for (uint32_t r = 0; r < reps; ++r) {
-                    const auto t0 = std::chrono::steady_clock::now();
+            for (uint32_t group : {2u, 4u, 8u, 16u})
+                for (uint32_t lanes : {1u, 2u}) {
+                    if (group > count || (lanes == 2 && 2 * group > count)) continue;
+                    if (base->lanes && lanes != base->lanes) continue;  // lanes fixed by the caller
+                    brp_options o = *base;
+                    o.engine = BRP_ENGINE_FUSED;
+                    o.packed = 1;
+                    o.snapshot = 0;
+                    o.batch = uint32_t(count);
+                    o.split_c = s.c;
+                    o.split_s1 = s.s1;
+                    o.split_s2 = s.s2;
+                    o.group = group;
+                    o.lanes = lanes;
+                    brp_hip* h = nullptr;
+                    if (int rc = brp_hip_create(&templates[0], input, input_count, &o, &h))
+                        throw Failure(rc, brp_last_error());
+                    std::unique_ptr<brp_hip> owner(h);
+                    if (int rc = brp_hip_set_templates(h, templates, count)) throw Failure(rc, brp_last_error());
                     launch(*h, 0);
-                    finish(*h, 0);
-                    us.push_back(std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() /
-                                 double(count));
+                    finish(*h, 0);  // warm-up (and graph capture)
+                    std::vector<double> us;
+                    for (uint32_t r = 0; r < reps; ++r) {
+                        const auto t0 = std::chrono::steady_clock::now();
+                        launch(*h, 0);
+                        finish(*h, 0);
+                        us.push_back(std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0)
+                                         .count() / double(count));
+                    }
+                    std::sort(us.begin(), us.end());
+                    const double med = us[us.size() / 2];
+                    log << "{\"C\":" << s.c << ",\"R\":" << s.r << ",\"S1\":" << s.s1 << ",\"S2\":" << s.s2
+                        << ",\"group\":" << group << ",\"lanes\":" << lanes << ",\"us_per_template\":" << med << "}\n";
+                    if (med < best_us) {
+                        best_us = med;
+                        best->split_c = s.c;
+                        best->split_s1 = s.s1;
+                        best->split_s2 = s.s2;
+                        best->group = group;
+                        best->lanes = lanes;
+                    }
                 }
-                std::sort(us.begin(), us.end());
-                const double med = us[us.size() / 2];
-                log << "{\"C\":" << s.c << ",\"R\":" << s.r << ",\"S1\":" << s.s1 << ",\"S2\":" << s.s2
-                    << ",\"group\":" << group << ",\"us_per_template\":" << med << "}\n";
-                if (med < best_us) {
-                    best_us = med;
-                    best->split_c = s.c;
-                    best->split_s1 = s.s1;
-                    best->split_s2 = s.s2;
-                    best->group = group;
-                }
-            }
         }
         const std::string text = log.str();
         if (report && cap) {

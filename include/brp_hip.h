@@ -82,7 +82,10 @@ typedef struct brp_options {
     uint32_t group;        /* fused engine: templates per launch (0 = 8) */
     uint32_t split_s1;     /* fused engine: pass-1 rows per block (0 = default for split_c) */
     uint32_t split_s2;     /* fused engine: pass-2 column pairs per block (0 = default) */
-    uint32_t reserved[3];
+    uint32_t lanes;        /* fused engine: groups in flight (1 or 2 streams; 0 = 1), each lane
+                              with its own exchange and power buffers */
+    uint32_t untimed;      /* fused engine: no per-kernel events (stage times read as 0) */
+    uint32_t reserved[1];
 } brp_options;
 
 /* src/vulkan.hpp:6 Timings; template_stage_ms is returned separately. */
@@ -183,10 +186,10 @@ int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches);
  * caller's `count` templates (one run of `count` per measurement, `reps` measurements
  * after a warm-up, median kept) for every compiled four-step split (pass-1 length C and
  * rows per block S1, pass-2 length R and column pairs per block S2: the LDS tiles, block
- * sizes and radix sequences) and every templates-per-launch value in {2, 4, 8, 16} not
- * above count.
- * *best receives `base` with split_c, split_s1, split_s2 and group set to the fastest; report
- * (may be NULL)
+ * sizes and radix sequences), every templates-per-launch value in {2, 4, 8, 16} not above
+ * count, and 1 or 2 lanes (only base->lanes when that is nonzero).
+ * *best receives `base` with split_c, split_s1, split_s2, group and lanes (1, 2) set to the
+ * fastest; report (may be NULL)
  * receives one JSON object per candidate, newline separated. */
 int brp_hip_autotune(const brp_params* templates, size_t count, const float* input, size_t input_count,
                      const brp_options* base, uint32_t reps, brp_options* best, char* report, size_t cap);

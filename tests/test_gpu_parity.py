@@ -163,9 +163,12 @@ def test_autotune_picks_a_correct_configuration(brp, wl):
     best, report = brp.autotune(w.templates, w.x, reps=2)
     say(best, report)
     shapes = {(r["C"], r["S1"], r["S2"]) for r in report}
-    assert len(report) == 3 * len(shapes) and len(shapes) >= 3  # every shape x group 2, 4, 8
+    # every shape x (group, lanes) in (2, 1), (2, 2), (4, 1), (4, 2), (8, 1)
+    assert len(report) == 5 * len(shapes) and len(shapes) >= 3
+    assert {(r["group"], r["lanes"]) for r in report} == {(2, 1), (2, 2), (4, 1), (4, 2), (8, 1)}
     assert {c for c, _, _ in shapes} == {4096, 2048, 1024}
     assert (best["split_c"], best["split_s1"], best["split_s2"]) in shapes and best["group"] in {2, 4, 8}
+    assert best["lanes"] in {1, 2}
     assert all(r["us_per_template"] > 0 for r in report)
     out = []
     for kw in ({}, best):
@@ -203,6 +206,26 @@ def test_fused_batch_slots_match_single_runs(brp, wl):
             g.set_template(p)
             g.run()
             assert np.array_equal(g.candidates(0), batched[s])
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_fused_two_lanes_match_one(brp, wl, graph):
+    """Two lanes (odd groups on a second stream with their own buffers) and untimed
+    graphs give the same candidates as one lane, on repeated runs."""
+    w = wl.bank(count=14, n=1 << 20)
+    say(f"[gpu] fused batch of 14, group 2/3, lanes 1 vs 2, graph={graph}, N=2^20")
+    ref = None
+    for lanes, group, untimed in ((1, 2, False), (2, 2, False), (2, 3, True), (2, 2, True)):
+        with brp.GPU(w.templates[0], w.x, 256, 0, 14, packed=True, engine=brp.ENGINE_FUSED, graph=graph,
+                     group=group, lanes=lanes, untimed=untimed) as g:
+            g.set_templates(w.templates)
+            for _ in range(3):
+                g.run()
+                got = [g.candidates(s) for s in range(14)]
+                if ref is None:
+                    ref = got
+                assert all(np.array_equal(a, b) for a, b in zip(ref, got)), (lanes, group, untimed)
+    assert sum(len(c) for c in ref) > 0
 
 
 def _oracle_bank(oracle, x, templates, orbits, rows=100):
