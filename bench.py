@@ -182,6 +182,26 @@ def main() -> int:
             [r.bin for r in top_p.final()] == final_bins
     log(f"[bench] plain: {plain_tps} templates/s over {plain_n}; toplist identical: {identical}")
 
+    # Bank top-1000 (1000 rows per level, final 1000) from both engines, outside the timed
+    # region.
+    top1000 = None
+    if plain_n == args.templates:
+        tables = []
+        for kw in (dict(packed=True, engine=brp.ENGINE_FUSED, graph=not args.no_graph, group=args.group,
+                        split_c=args.split_c, split_s1=args.split_s1, split_s2=args.split_s2),
+                   dict(stockham=True, packed=True, scatter=True, radix8=True, fused_power=True, radix8_store=True,
+                        engine=brp.ENGINE_PLAIN)):
+            batch = args.batch if kw["engine"] == brp.ENGINE_FUSED else 1
+            with brp.GPU(w.templates[0], w.x, 256, 0, batch, **kw) as gk:
+                t1k = brp.Toplist(1000)
+                gk.bank_search(w.orbits, U, workloads.SAMPLE_US, t1k)
+                tables.append(([(r.bin, r.harmonics, r.period) for r in t1k.table()],
+                               [r.power for r in t1k.table()], [r.bin for r in t1k.final()]))
+        (ra, pa, fa), (rb, pb, fb) = tables
+        top1000 = {"rows": sum(1 for x in pa if x > 0), "final": len(fa),
+                   "identical": ra == rb and bool(np.allclose(pa, pb, rtol=1e-5, atol=0)) and fa == fb}
+        log(f"[bench] top-1000: {top1000}")
+
     # --------------------------------------------------------------- roofline --
     B = workloads.algorithmic_bytes(n, U, H)
     # Per-kernel algorithmic bytes per template: pass 1 reads the series and writes the
@@ -197,7 +217,7 @@ def main() -> int:
     pipeline_achieved = B * templates_per_s
 
     # ------------------------------------------------------------ CPU baseline --
-    cpu = None
+    cpu, power_error = None, None
     if not args.no_cpu:
         import oracle_lib
 
@@ -210,11 +230,26 @@ def main() -> int:
         reps = max(1, int(args.cpu_seconds / max(spent, 1e-3)) - 1)
         if reps > 1:
             ms = oracle_lib.benchmark_cpu(w.x, sample, threads, min(reps, 5))
+        ms1 = oracle_lib.benchmark_cpu(w.x, w.templates[:1], 1, 3)
         cpu = {"value": round(len(sample) / (ms * 1e-3), 3), "unit": "templates/s", "cores": threads,
                "kind": "port",
                "sample": f"{len(sample)} bank templates at N={n} (one per thread), resample + FP32 Stockham FFT "
-                         f"(not FFTW) + power + harmonic + select, median of repetitions after one warm-up"}
+                         f"(not FFTW) + power + harmonic + select, median of repetitions after one warm-up",
+               "single_core": {"value": round(1.0 / (ms1 * 1e-3), 3), "unit": "templates/s", "cores": 1,
+                               "sample": "bank template 0, median of 3 after one warm-up"}}
         log(f"[bench] cpu baseline {cpu}")
+        # Power error of the fused engine against the oracle's FP64 FFT (template 0).
+        import parity
+        with brp.GPU(w.templates[0], w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True,
+                     split_c=args.split_c, split_s1=args.split_s1, split_s2=args.split_s2) as gs:
+            gs.set_template(w.templates[0])
+            gs.run()
+            snap = gs.snapshot(0)
+        p64 = oracle_lib.reference(w.x, w.templates[0])[3]
+        pe = parity.power_error(snap.power, p64)
+        power_error = {"max_rel": pe["max_rel"], "rms_rel": pe["rms_rel"],
+                       "definition": "|P_gpu - P_fp64| / max(P_fp64, mean P_fp64), bank template 0"}
+        log(f"[bench] power error vs FP64 oracle {power_error}")
 
     out = {
         "metric": "templates/sec at N=2^22, batched vs plain (1 GPU), identical toplist; % roofline",
@@ -240,13 +275,18 @@ def main() -> int:
         "plain_templates_per_s": round(plain_tps, 3) if plain_tps else None,
         "batched_over_plain": round(templates_per_s / plain_tps, 3) if plain_tps else None,
         "toplist_identical": identical,
+        "top1000": top1000,
+        "power_error": power_error,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
                      "per_launch_ms": round(per_launch_ms, 4), "templates_per_launch": round(templates_per_launch, 3),
                      "algorithmic_bytes_per_template": kb[dom]},
         "pipeline_roofline": {"bytes_per_template": B, "achieved_GBps": round(pipeline_achieved / 1e9, 2),
-                              "frac": round(pipeline_achieved / HBM_PEAK, 4)},
+                              "frac": round(pipeline_achieved / HBM_PEAK, 4),
+                              "stage_boundary_bytes_per_template": 4 * U + 18 * n + 4 * H,
+                              "fft_flop_per_template": round(5 * (n // 2) * np.log2(n // 2)),
+                              "fft_tflops": round(5 * (n // 2) * np.log2(n // 2) * templates_per_s / 1e12, 3)},
         "kernel_us_per_template": {k: round(v * 1e3 / (args.templates * args.steps), 3) for k, v in kms.items()},
         "copy_GBps": round(copy_gbps, 1),
         "cpu_baseline": cpu,
