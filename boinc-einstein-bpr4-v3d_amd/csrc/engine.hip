@@ -246,7 +246,9 @@ void create_fused(brp_hip& h) {
     h.lane_p = size_t(h.group) * s.m;
     h.lane_partial = size_t(h.group) * s.blocks1;
     a.Y = h.alloc<float2>(h.lanes * h.lane_y);
-    a.P = h.alloc<float>(h.lanes * h.lane_p);
+    // 2048 floats past the last slot: the harmonic kernel reads the dyadic-sum sources of
+    // its last tile's bins unconditionally (up to H + 1031 < M + 2048) and masks them.
+    a.P = h.alloc<float>(h.lanes * h.lane_p + 2048);
     a.partial = h.alloc<double>(h.lanes * h.lane_partial);
     if (h.lanes == 2) {
         HIPC(hipStreamCreateWithFlags(&h.side, hipStreamNonBlocking));

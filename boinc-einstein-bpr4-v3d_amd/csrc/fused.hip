@@ -475,41 +475,57 @@ __global__ __launch_bounds__(kHarmThreads) void k_harmonic(FusedArgs a, FusedSha
 
     // Phase A: dyadic sums of the tile's bins i0 .. i0 + kTile - 1 (LDS slots 8 ..) and
     // of the 8 bins before it (slots 0 .. 7), -inf outside [W, H); level-0 candidates.
+    // Every source value of the thread's bins is read first, unconditionally (the power
+    // buffer has room past the last bin a tile can name), so all loads are in flight
+    // together; bins outside [W, H) are masked afterwards.
     {
         const uint32_t ib = uint32_t(i0 + t);
         uint32_t idx[16];
 #pragma unroll
         for (int k = 1; k <= 16; ++k) idx[k - 1] = (__umul24(ib, uint32_t(k)) + 8u) >> 4;
+        float val[kPerThread][16];
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u)
+#pragma unroll
+            for (int k = 1; k <= 16; ++k) val[u][k - 1] = P[idx[k - 1] + uint32_t(16 * k * u)];
+        const bool halo = t < 8;
+        const int ih = i0 - 8 + t;
+        float hval[16];
+        if (halo) {
+            const uint32_t ic = uint32_t(max(ih, 0));
+#pragma unroll
+            for (int k = 1; k <= 16; ++k) hval[k - 1] = P[(__umul24(ic, uint32_t(k)) + 8u) >> 4];
+        }
+        // Historical order (shaders/hs_sum.glsl:2-10); at(k) = P[(i k + 8) >> 4].
+        auto dyadic = [](const float* at, float s[4]) {
+            float v = at[15];
+            v += at[7]; s[0] = v;
+            v += at[11] + at[3]; s[1] = v;
+            v += at[13] + at[9] + at[5] + at[1]; s[2] = v;
+            v += at[14] + at[12] + at[10] + at[8] + at[6] + at[4] + at[2] + at[0]; s[3] = v;
+        };
 #pragma unroll
         for (int u = 0; u < kPerThread; ++u) {
             const int i = i0 + t + kHarmThreads * u;
-            float s[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
-            if (i >= W && i < H) {
-                auto at = [&](int k) { return P[idx[k - 1] + uint32_t(16 * k * u)]; };
-                float v = at(16);
-                v += at(8); s[0] = v;
-                v += at(12) + at(4); s[1] = v;
-                v += at(14) + at(10) + at(6) + at(2); s[2] = v;
-                v += at(15) + at(13) + at(11) + at(9) + at(7) + at(5) + at(3) + at(1); s[3] = v;
-            }
+            float s[4];
+            dyadic(val[u], s);
+            const bool live = i >= W && i < H;
             const int d = sum_at(8 + t + kHarmThreads * u);
-            sums[0][d] = s[0]; sums[1][d] = s[1]; sums[2][d] = s[2]; sums[3][d] = s[3];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) sums[h][d] = live ? s[h] : -INFINITY;
             if (i < F) {
-                const float v0 = P[i];
+                const float v0 = val[u][15];  // P[i]
                 if constexpr (SNAP) snap[i] = v0;
                 if (i >= W && v0 > thr0) append_candidate(res, p.capacity, uint32_t(i), 0, v0);
             }
         }
-        if (t < 8) {
-            const int i = i0 - 8 + t;
-            float s[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
-            if (i >= W && i < H) {
-                float full[5];
-                dyadic_sums(P, uint32_t(i), full);
-                s[0] = full[1]; s[1] = full[2]; s[2] = full[3]; s[3] = full[4];
-            }
+        if (halo) {
+            float s[4];
+            dyadic(hval, s);
+            const bool live = ih >= W && ih < H;
             const int d = sum_at(t);
-            sums[0][d] = s[0]; sums[1][d] = s[1]; sums[2][d] = s[2]; sums[3][d] = s[3];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) sums[h][d] = live ? s[h] : -INFINITY;
         }
     }
     __syncthreads();
