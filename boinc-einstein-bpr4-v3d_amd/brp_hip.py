@@ -119,6 +119,9 @@ def load() -> ctypes.CDLL:
         "brp_hip_profile_counters": (sz, [ctypes.c_void_p, ctypes.c_char_p, sz]),
         "brp_hip_kernel_ms": (i32, [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_uint64)]),
         "brp_hip_copy_bandwidth": (i32, [sz, i32, P(ctypes.c_double)]),
+        "brp_hip_add_series": (i32, [ctypes.c_void_p, P(Params), fp, sz]),
+        "brp_hip_bank_search_templates": (i32, [ctypes.c_void_p, P(Params), P(Orbit), sz, P(ctypes.c_void_p), sz,
+                                                P(u32), P(f64)]),
         "brp_hip_autotune": (i32, [P(Params), sz, fp, sz, P(Options), u32, P(Options), ctypes.c_char_p, sz]),
         "brp_hip_bank_search": (i32, [ctypes.c_void_p, P(Orbit), sz, u32, f64, f64, f64, u32, ctypes.c_void_p,
                                       P(Params), P(u32), P(ctypes.c_double)]),
@@ -379,6 +382,23 @@ class GPU:
         n = ctypes.c_uint64()
         _check(load().brp_hip_kernel_ms(self._h, out, ctypes.byref(n)))
         return list(out), n.value
+
+    def add_series(self, p: Params, input: np.ndarray) -> None:
+        """Registers another observation (fused engine): templates name it by (n, unpadded)."""
+        x = np.ascontiguousarray(input, dtype=np.float32)
+        _check(load().brp_hip_add_series(self._h, ctypes.byref(p), _fptr(x), x.size))
+
+    def bank_search_templates(self, templates, orbits, tops):
+        """Bank loop over prepared templates of any registered series, tops[i] the toplist of
+        series i; returns (counts, gpu_ms)."""
+        tarr = (Params * len(templates))(*templates)
+        oarr = (Orbit * len(orbits))(*[o if isinstance(o, Orbit) else Orbit(*o) for o in orbits])
+        harr = (ctypes.c_void_p * len(tops))(*[tp.handle for tp in tops])
+        counts = (ctypes.c_uint32 * len(tarr))()
+        gpu_ms = ctypes.c_double()
+        _check(load().brp_hip_bank_search_templates(self._h, tarr, oarr, len(tarr), harr, len(tops), counts,
+                                                    ctypes.byref(gpu_ms)))
+        return np.frombuffer(counts, dtype=np.uint32).copy(), gpu_ms.value
 
     def bank_search(self, orbits, samples: int, sample_us: float, top: Toplist, f0: float = 400,
                     false_alarm: float = 0.08, window: int = 1000):

@@ -56,7 +56,7 @@ struct FusedArgs {
     const float2* tw_rq = nullptr;   // W_{16 R}^t, t < 16 R
     const float2* twn_lo = nullptr;  // W_N^t, t < 2048
     const float2* twn_hi = nullptr;  // W_N^{2048 q}
-    float2* Y = nullptr;             // per group slot: M complex
+    float2* Y = nullptr;             // per group slot: M complex (stride M of the group's series)
     float* P = nullptr;              // per group slot: M floats
     double* partial = nullptr;       // per group slot: blocks1 doubles
     float* mean = nullptr;           // per group slot
@@ -72,7 +72,9 @@ struct FusedArgs {
     int32_t* snap_index = nullptr;    // N int32
     float2* snap_X = nullptr;         // M + 1 complex
     float* snap_summed = nullptr;     // 5 F floats
-    uint32_t snap_f = 0;
+    uint32_t snap_f = 0;              // summed stride (largest F)
+    uint32_t snap_n = 0;              // resampled / index stride (largest N)
+    uint32_t snap_m1 = 0;             // spectrum stride (largest M + 1)
 };
 
 // The shape of series length n with pass-1 length c, pass-1 rows per block s1 and pass-2

@@ -88,7 +88,7 @@ struct brp_hip {
     brp_params base{};
     brp_options opt{};
     hipStream_t stream = nullptr;
-    std::vector<void*> allocations;
+    std::vector<std::pair<void*, size_t>> allocations;
     uint64_t allocated = 0;
     // Two halves of slots, parameters, staging and graph: the public calls use half 0; the
     // bank loop alternates halves so that one batch runs while the previous is retained.
@@ -108,19 +108,34 @@ struct brp_hip {
     brp_engine::PlainBuffers pb;
     float* power_buf = nullptr;  // n/2 power + 5 F summed (the reference's buffer 4 layout)
 
-    // fused engine
-    brp_engine::FusedShape shape;
-    brp_engine::FusedArgs fa;
+    // fused engine: one record per observation (series); templates pick theirs by
+    // (n, unpadded).  Exchange / power / partial buffers are shared, sized for the
+    // largest series.
+    struct Series {
+        uint32_t n = 0, unpadded = 0;
+        brp_engine::FusedShape shape;
+        brp_engine::FusedArgs tables;  // x, lut and the twiddle tables of this length
+    };
+    std::vector<Series> series;
+    brp_engine::FusedArgs fa;        // shared buffers and settings
     uint32_t group = 1;
-    uint32_t tiles = 0;
     uint32_t lanes = 1;              // groups in flight, on stream and side
     bool timed = true;               // per-kernel events
     hipStream_t side = nullptr;      // second lane
     hipEvent_t fork = nullptr, join = nullptr;
     size_t lane_y = 0, lane_p = 0, lane_partial = 0;  // per-lane buffer strides (elements)
+    uint32_t m_max = 0, blocks1_max = 0, n_max = 0;   // what the shared buffers hold
+    // A launch group: `count` consecutive slots from `first`, one series, one index form.
+    struct Group {
+        uint32_t first, count, series, nonneg, tiles;
+        bool operator==(const Group& o) const {
+            return first == o.first && count == o.count && series == o.series && nonneg == o.nonneg &&
+                   tiles == o.tiles;
+        }
+    };
+    std::vector<Group> groups[2];    // of the last launch of each half
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-    uint32_t graph_count[2] = {0, 0};
-    std::vector<uint8_t> graph_nonneg[2];  // per group of the captured graph
+    std::vector<Group> graph_groups[2];  // what each captured graph runs
     bool graph_events = true;
 
     std::vector<hipEvent_t> events[2];
@@ -139,9 +154,19 @@ struct brp_hip {
         void* p = nullptr;
         const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
         HIPC(hipMalloc(&p, bytes));
-        allocations.push_back(p);
+        allocations.emplace_back(p, bytes);
         allocated += bytes;
         return static_cast<T*>(p);
+    }
+    void release(void* p) {
+        if (!p) return;
+        for (auto it = allocations.begin(); it != allocations.end(); ++it)
+            if (it->first == p) {
+                HIPC(hipFree(p));
+                allocated -= it->second;
+                allocations.erase(it);
+                return;
+            }
     }
     template <class T>
     T* upload(const std::vector<T>& v) {
@@ -162,7 +187,7 @@ struct brp_hip {
         for (auto g : graph_exec) if (g) hipGraphExecDestroy(g);
         for (auto& list : events) for (hipEvent_t e : list) hipEventDestroy(e);
         for (hipEvent_t e : done) if (e) hipEventDestroy(e);
-        for (void* p : allocations) hipFree(p);
+        for (auto& a : allocations) hipFree(a.first);
         if (staging_host) hipHostFree(staging_host);
         if (params_host) hipHostFree(params_host);
         if (fork) hipEventDestroy(fork);
@@ -223,54 +248,121 @@ void create_plain(brp_hip& h) {
     HIPC(hipMemset(h.power_buf, 0, sizeof(float) * (size_t(p.n / 2) + 5ull * p.fundamental)));
 }
 
-void create_fused(brp_hip& h) {
-    const brp_params& p = h.base;
-    require(brp_engine::fused_supported(p.n, h.opt.split_c, h.opt.split_s1, h.opt.split_s2, &h.shape),
-            "fused engine: unsupported series length (N = 2^13..2^23 or 3*2^15..3*2^22)");
-    const auto& s = h.shape;
-    h.group = h.opt.snapshot ? h.opt.batch : std::min<uint32_t>(h.opt.batch, h.opt.group ? h.opt.group : 8);
+// Invalidates the captured graphs (they hold buffer addresses and launch plans).
+void drop_graphs(brp_hip& h) {
+    for (int half = 0; half < 2; ++half) {
+        if (h.graph_exec[half]) HIPC(hipGraphExecDestroy(h.graph_exec[half]));
+        h.graph_exec[half] = nullptr;
+        h.graph_groups[half].clear();
+    }
+}
+
+// (Re)allocates the shared fused buffers for the largest series: per lane `group` slots
+// of exchange (M complex), power (M floats) and pass-1 partial sums; snapshots per batch
+// slot at the largest N.
+void size_fused_buffers(brp_hip& h) {
+    uint32_t m = 0, b1 = 0, n = 0;
+    for (const auto& s : h.series) {
+        m = std::max(m, s.shape.m);
+        b1 = std::max(b1, s.shape.blocks1);
+        n = std::max(n, s.n);
+    }
+    if (m <= h.m_max && b1 <= h.blocks1_max && n <= h.n_max) return;
+    drop_graphs(h);
     auto& a = h.fa;
-    a.x = h.x;
-    a.lut = h.lut;
-    a.tw_c = h.upload(twiddles(s.c, s.c));
-    a.tw_r = h.upload(twiddles(s.r, s.r));
-    a.twm_lo = h.upload(twiddles(s.m, 2048));
-    a.twm_hi = h.upload(twiddles(s.m, (s.m + 2047) / 2048, 2048));
-    a.tw_rq = h.upload(twiddles(16ull * s.r, 16ull * s.r));
-    a.twn_lo = h.upload(twiddles(s.n, 2048));
-    a.twn_hi = h.upload(twiddles(s.n, (s.n + 2047) / 2048, 2048));
-    h.lanes = h.opt.snapshot ? 1 : (h.opt.lanes ? h.opt.lanes : 1);
-    require(h.lanes <= 2, "fused engine: lanes must be 1 or 2");
-    h.timed = !h.opt.untimed;
-    h.lane_y = size_t(h.group) * s.m;
-    h.lane_p = size_t(h.group) * s.m;
-    h.lane_partial = size_t(h.group) * s.blocks1;
+    h.release(a.Y);
+    h.release(a.P);
+    h.release(a.partial);
+    h.m_max = m;
+    h.blocks1_max = b1;
+    h.n_max = n;
+    h.lane_y = size_t(h.group) * m;
+    h.lane_p = size_t(h.group) * m;
+    h.lane_partial = size_t(h.group) * b1;
     a.Y = h.alloc<float2>(h.lanes * h.lane_y);
     // 2048 floats past the last slot: the harmonic kernel reads the dyadic-sum sources of
     // its last tile's bins unconditionally (up to H + 1031 < M + 2048) and masks them.
     a.P = h.alloc<float>(h.lanes * h.lane_p + 2048);
     a.partial = h.alloc<double>(h.lanes * h.lane_partial);
-    if (h.lanes == 2) {
-        HIPC(hipStreamCreateWithFlags(&h.side, hipStreamNonBlocking));
-        HIPC(hipEventCreateWithFlags(&h.fork, hipEventDisableTiming));
-        HIPC(hipEventCreateWithFlags(&h.join, hipEventDisableTiming));
+    if (h.opt.snapshot) {
+        h.release(a.snap_resampled);
+        h.release(a.snap_index);
+        h.release(a.snap_X);
+        h.release(a.snap_summed);
+        a.snap_n = n;
+        a.snap_m1 = m + 1;
+        a.snap_f = n / 2;
+        a.snap_resampled = h.alloc<float>(size_t(h.opt.batch) * a.snap_n);
+        a.snap_index = h.alloc<int32_t>(size_t(h.opt.batch) * a.snap_n);
+        a.snap_X = h.alloc<float2>(size_t(h.opt.batch) * a.snap_m1);
+        a.snap_summed = h.alloc<float>(size_t(h.opt.batch) * 5 * a.snap_f);
     }
+}
+
+// Registers an observation of n padded / unpadded samples already on the device (with
+// four zero guard samples after it): its four-step shape (the option overrides where
+// they fit this length, else the default) and twiddle tables.
+void add_fused_series(brp_hip& h, const brp_params& p, float* x) {
+    for (const auto& s : h.series)
+        require(!(s.n == p.n && s.unpadded == p.unpadded), "a series of this length is already registered");
+    brp_hip::Series s;
+    s.n = p.n;
+    s.unpadded = p.unpadded;
+    if (!brp_engine::fused_supported(p.n, h.opt.split_c, h.opt.split_s1, h.opt.split_s2, &s.shape))
+        require(!h.series.empty() && brp_engine::fused_supported(p.n, 0, 0, 0, &s.shape),
+                "fused engine: unsupported series length (N = 2^13..2^23 or 3*2^15..3*2^22) or split");
+    const auto& sh = s.shape;
+    auto& a = s.tables;
+    a.x = x;
+    a.lut = h.lut;
+    a.tw_c = h.upload(twiddles(sh.c, sh.c));
+    a.tw_r = h.upload(twiddles(sh.r, sh.r));
+    a.twm_lo = h.upload(twiddles(sh.m, 2048));
+    a.twm_hi = h.upload(twiddles(sh.m, (sh.m + 2047) / 2048, 2048));
+    a.tw_rq = h.upload(twiddles(16ull * sh.r, 16ull * sh.r));
+    a.twn_lo = h.upload(twiddles(sh.n, 2048));
+    a.twn_hi = h.upload(twiddles(sh.n, (sh.n + 2047) / 2048, 2048));
+    h.series.push_back(s);
+    size_fused_buffers(h);
+}
+
+void create_fused(brp_hip& h) {
+    h.group = h.opt.snapshot ? h.opt.batch : std::min<uint32_t>(h.opt.batch, h.opt.group ? h.opt.group : 8);
+    h.lanes = h.opt.snapshot ? 1 : (h.opt.lanes ? h.opt.lanes : 1);
+    require(h.lanes <= 2, "fused engine: lanes must be 1 or 2");
+    h.timed = !h.opt.untimed;
+    auto& a = h.fa;
     a.mean = h.alloc<float>(2 * size_t(h.opt.batch));
     a.results = h.results;
     a.slot_words = h.slot_words;
     a.params = h.params_dev;
     a.fp64_phase = h.opt.fp64_phase;
-    if (h.opt.snapshot) {
-        a.snap_resampled = h.alloc<float>(size_t(h.opt.batch) * s.n);
-        a.snap_index = h.alloc<int32_t>(size_t(h.opt.batch) * s.n);
-        a.snap_X = h.alloc<float2>(size_t(h.opt.batch) * (s.m + 1));
-        a.snap_summed = h.alloc<float>(size_t(h.opt.batch) * 5 * p.fundamental);
-        a.snap_f = p.fundamental;
+    if (h.lanes == 2) {
+        HIPC(hipStreamCreateWithFlags(&h.side, hipStreamNonBlocking));
+        HIPC(hipEventCreateWithFlags(&h.fork, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&h.join, hipEventDisableTiming));
     }
-    // Harmonic tiles cover every fundamental's rounding interval: centres j 2^h < 16 F,
-    // intervals that start below H.
-    const uint64_t reach = std::min<uint64_t>(16ull * p.fundamental, uint64_t(p.harmonic) + 8);
-    h.tiles = uint32_t((reach + 1023) / 1024);
+    add_fused_series(h, h.base, h.x);
+}
+
+// Series of a template: the registered observation with its (n, unpadded).
+uint32_t series_of(const brp_hip& h, const brp_params& p) {
+    for (uint32_t i = 0; i < h.series.size(); ++i)
+        if (h.series[i].n == p.n && h.series[i].unpadded == p.unpadded) return i;
+    throw Failure(BRP_E_INVALID, "template dimensions changed; rebuild resources (no series of this length)");
+}
+
+// The fixed-dimension rule of src/vulkan.cpp:270-276, lifted for the fused engine: a
+// template may name any registered series (its n and unpadded), with its own F and H;
+// the result capacity and the arithmetic flags stay those of the handle.
+void check_template(const brp_hip& h, const brp_params& next) {
+    if (h.opt.engine == BRP_ENGINE_PLAIN) return check_template(h.base, next);
+    require((next.flags & 6) == (h.base.flags & 6), "arithmetic specialization changed; rebuild pipelines");
+    (void)series_of(h, next);
+    require(next.capacity == h.base.capacity && next.steps && next.steps <= next.unpadded,
+            "template dimensions changed; rebuild resources");
+    require(next.window <= next.fundamental && next.harmonic <= next.n / 2 && next.fundamental <= next.n / 2,
+            "template search range outside the spectrum");
 }
 
 // The plain launch sequence for slot r (src/vulkan.cpp:221-266), with events at the
@@ -293,26 +385,40 @@ void enqueue_plain(brp_hip& h, uint32_t r) {
     HIPC(hipEventRecord(h.event(0, 5 * r + 4), st));
 }
 
-// Per group of `half`: whether every template's orbital phase is non-negative for all
-// samples (omega, psi, dt >= 0), which selects the shorter exact index form.
-std::vector<uint8_t> group_nonneg(const brp_hip& h, int half, uint32_t count) {
-    std::vector<uint8_t> out;
+// The launch groups of `half`: runs of consecutive slots of one series, cut into at
+// most `group` templates.  Per group: whether every template's orbital phase is
+// non-negative for all samples (omega, psi, dt >= 0, which selects the shorter exact
+// index form) and the harmonic tiles that cover every fundamental's rounding interval
+// (centres j 2^h < 16 F, intervals that start below H).
+std::vector<brp_hip::Group> plan_groups(const brp_hip& h, int half) {
+    std::vector<brp_hip::Group> out;
     const auto& tp = h.templates[half];
-    for (uint32_t first = 0; first < count; first += h.group) {
+    const uint32_t count = uint32_t(tp.size());
+    uint32_t first = 0;
+    while (first < count) {
+        const uint32_t s = series_of(h, tp[first]);
+        uint32_t end = first + 1;
+        while (end < count && end - first < h.group && series_of(h, tp[end]) == s) ++end;
         bool nn = true;
-        for (uint32_t i = first; i < std::min(count, first + h.group); ++i)
+        uint64_t reach = 0;
+        for (uint32_t i = first; i < end; ++i) {
             nn = nn && tp[i].omega >= 0.0f && tp[i].psi >= 0.0f && tp[i].dt >= 0.0f;
-        out.push_back(nn ? 1 : 0);
+            reach = std::max(reach, std::min<uint64_t>(16ull * tp[i].fundamental, uint64_t(tp[i].harmonic) + 8));
+        }
+        out.push_back({first, end - first, s, nn ? 1u : 0u, uint32_t((reach + 1023) / 1024)});
+        first = end;
     }
     return out;
 }
 
-// The fused sequence for `count` templates in slots of `half`: every group of `group`
-// templates in three launches, then the packing of the results for one readback.  With
-// two lanes, even groups run on the handle's stream and odd groups on the side stream,
-// each lane with its own exchange / power / partial buffers, so one group's launches
-// overlap the other's (and the gaps between them).
-void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
+// The fused sequence for the templates of `half`: every launch group in three launches,
+// then the packing of the results for one readback.  With two lanes, even groups run on
+// the handle's stream and odd groups on the side stream, each lane with its own
+// exchange / power / partial buffers, so one group's launches overlap the other's (and
+// the gaps between them).
+void enqueue_fused(brp_hip& h, int half, bool capturing) {
+    const auto& groups = h.groups[half];
+    const uint32_t count = uint32_t(h.templates[half].size());
     const size_t base = h.slot_base(half);
     uint32_t* results = h.results + base * h.slot_words;
     brp_engine::plain_clear_slots(h.stream, results, h.slot_words, count);
@@ -327,25 +433,36 @@ void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
         if (capturing) HIPC(hipEventRecordWithFlags(h.event(half, i), st, hipEventRecordExternal));
         else HIPC(hipEventRecord(h.event(half, i), st));
     };
-    const std::vector<uint8_t> nonneg = group_nonneg(h, half, count);
-    uint32_t gi = 0;
-    for (uint32_t first = 0; first < count; first += h.group, ++gi) {
+    for (uint32_t gi = 0; gi < groups.size(); ++gi) {
+        const auto& grp = groups[gi];
+        const auto& ser = h.series[grp.series];
         brp_engine::FusedArgs a = h.fa;
+        const auto& tb = ser.tables;
+        a.x = tb.x;
+        a.lut = tb.lut;
+        a.tw_c = tb.tw_c;
+        a.tw_r = tb.tw_r;
+        a.twm_lo = tb.twm_lo;
+        a.twm_hi = tb.twm_hi;
+        a.tw_rq = tb.tw_rq;
+        a.twn_lo = tb.twn_lo;
+        a.twn_hi = tb.twn_hi;
         const uint32_t lane = gi % h.lanes;
         st = lane ? h.side : h.stream;
         a.Y += lane * h.lane_y;
         a.P += lane * h.lane_p;
         a.partial += lane * h.lane_partial;
-        a.first = uint32_t(base) + first;
-        a.count = std::min(h.group, count - first);
+        if (h.opt.snapshot) a.P += size_t(grp.first) * h.m_max;  // snapshots keep every slot's power
+        a.first = uint32_t(base) + grp.first;
+        a.count = grp.count;
         a.fma = (h.base.flags & 2u) ? 1u : 0u;
-        a.nonneg = nonneg[gi];
+        a.nonneg = grp.nonneg;
         record(4 * gi);
-        brp_engine::fused_pass1(st, h.shape, a);
+        brp_engine::fused_pass1(st, ser.shape, a);
         record(4 * gi + 1);
-        brp_engine::fused_pass2(st, h.shape, a);
+        brp_engine::fused_pass2(st, ser.shape, a);
         record(4 * gi + 2);
-        brp_engine::fused_harmonic(st, h.shape, a, h.tiles);
+        brp_engine::fused_harmonic(st, ser.shape, a, grp.tiles);
         record(4 * gi + 3);
     }
     if (h.lanes == 2) {
@@ -356,20 +473,20 @@ void enqueue_fused(brp_hip& h, int half, uint32_t count, bool capturing) {
     brp_engine::pack_results(st, results, h.slot_words, count, h.keep, h.staging_dev + base * (h.keep + 1) * 4);
 }
 
-void run_fused(brp_hip& h, int half, uint32_t count) {
+void run_fused(brp_hip& h, int half) {
+    h.groups[half] = plan_groups(h, half);
     if (!h.opt.graph || h.opt.snapshot) {
-        enqueue_fused(h, half, count, false);
+        enqueue_fused(h, half, false);
         return;
     }
-    const std::vector<uint8_t> nonneg = group_nonneg(h, half, count);
-    if (!h.graph_exec[half] || h.graph_count[half] != count || h.graph_nonneg[half] != nonneg) {
+    if (!h.graph_exec[half] || h.graph_groups[half] != h.groups[half]) {
         if (h.graph_exec[half]) { HIPC(hipGraphExecDestroy(h.graph_exec[half])); h.graph_exec[half] = nullptr; }
         hipGraph_t graph = nullptr;
         auto capture = [&](bool events) {
             h.graph_events = events;
             HIPC(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
             try {
-                enqueue_fused(h, half, count, true);
+                enqueue_fused(h, half, true);
             } catch (...) {
                 hipGraph_t dropped = nullptr;
                 hipStreamEndCapture(h.stream, &dropped);
@@ -386,8 +503,7 @@ void run_fused(brp_hip& h, int half, uint32_t count) {
         }
         HIPC(hipGraphInstantiate(&h.graph_exec[half], graph, nullptr, nullptr, 0));
         HIPC(hipGraphDestroy(graph));
-        h.graph_count[half] = count;
-        h.graph_nonneg[half] = nonneg;
+        h.graph_groups[half] = h.groups[half];
     }
     HIPC(hipGraphLaunch(h.graph_exec[half], h.stream));
 }
@@ -408,9 +524,9 @@ void fill_timings(brp_hip& h, int half, uint32_t count) {
         return;
     }
     if (!h.timed || (h.opt.graph && !h.opt.snapshot && !h.graph_events)) return;
-    const uint32_t groups = (count + h.group - 1) / h.group;
-    for (uint32_t gi = 0; gi < groups; ++gi) {
-        const uint32_t first = gi * h.group, n = std::min(h.group, count - first);
+    const auto& groups = h.groups[half];
+    for (uint32_t gi = 0; gi < groups.size(); ++gi) {
+        const uint32_t first = groups[gi].first, n = groups[gi].count;
         double stage[4] = {0, 0, 0, 0};
         HIPC(hipEventElapsedTime(&ms, h.event(half, 4 * gi), h.event(half, 4 * gi + 1)));
         stage[0] = ms;
@@ -446,9 +562,9 @@ void launch(brp_hip& h, int half) {
         if (h.opt.snapshot) {
             require(half == 0, "snapshots are kept for one batch at a time");
             HIPC(hipMemsetAsync(h.fa.snap_summed, 0, sizeof(float) * size_t(count) * 5 * h.fa.snap_f, h.stream));
-            HIPC(hipMemsetAsync(h.fa.snap_X, 0, sizeof(float2) * size_t(count) * (h.shape.m + 1), h.stream));
+            HIPC(hipMemsetAsync(h.fa.snap_X, 0, sizeof(float2) * size_t(count) * h.fa.snap_m1, h.stream));
         }
-        run_fused(h, half, count);
+        run_fused(h, half);
     }
     h.submit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - submit0).count();
     HIPC(hipMemcpyAsync(h.staging_host + base * (h.keep + 1) * 4, h.staging_dev + base * (h.keep + 1) * 4,
@@ -464,7 +580,7 @@ void finish(brp_hip& h, int half) {
     const uint32_t count = uint32_t(h.templates[half].size());
     fill_timings(h, half, count);
     for (int s = 0; s < 4; ++s) h.total_ms[s] += h.kernel_ms[half][s];
-    h.total_launches += h.opt.engine == BRP_ENGINE_PLAIN ? count : (count + h.group - 1) / h.group;
+    h.total_launches += h.opt.engine == BRP_ENGINE_PLAIN ? count : h.groups[half].size();
 }
 
 void read_slot(brp_hip& h, int half, uint32_t slot, std::vector<brp_candidate>& out) {
@@ -542,7 +658,7 @@ int brp_hip_set_templates(brp_hip* h, const brp_params* p, size_t count) {
     return guarded([&] {
         require(h && p, "null argument");
         require(count && count <= h->opt.batch, "template batch exceeds retained result capacity");
-        for (size_t i = 0; i < count; ++i) check_template(h->base, p[i]);
+        for (size_t i = 0; i < count; ++i) check_template(*h, p[i]);
         h->templates[0].assign(p, p + count);
         h->ran[0] = false;
     });
@@ -606,17 +722,29 @@ int brp_hip_snapshot(brp_hip* h, uint32_t slot, float* resampled, int32_t* index
         require(h->opt.snapshot, "fused engine snapshots need options.snapshot");
         const auto& a = h->fa;
         if (resampled) {
-            HIPC(hipMemcpy(resampled, a.snap_resampled + size_t(slot) * n, sizeof(float) * p.steps, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(resampled, a.snap_resampled + size_t(slot) * a.snap_n, sizeof(float) * p.steps,
+                           hipMemcpyDeviceToHost));
             float mean = 0;
             HIPC(hipMemcpy(&mean, a.mean + slot, sizeof(float), hipMemcpyDeviceToHost));
             std::fill(resampled + p.steps, resampled + n, mean);
         }
-        if (index) HIPC(hipMemcpy(index, a.snap_index + size_t(slot) * n, sizeof(int32_t) * p.steps, hipMemcpyDeviceToHost));
-        if (fft) HIPC(hipMemcpy(fft, a.snap_X + size_t(slot) * (half + 1), sizeof(float2) * (half + 1), hipMemcpyDeviceToHost));
-        if (power) HIPC(hipMemcpy(power, a.P + size_t(slot) * h->shape.m, sizeof(float) * half, hipMemcpyDeviceToHost));
-        if (summed)
-            HIPC(hipMemcpy(summed, a.snap_summed + size_t(slot) * 5 * a.snap_f, sizeof(float) * 5ull * p.fundamental,
-                           hipMemcpyDeviceToHost));
+        if (index)
+            HIPC(hipMemcpy(index, a.snap_index + size_t(slot) * a.snap_n, sizeof(int32_t) * p.steps, hipMemcpyDeviceToHost));
+        if (fft)
+            HIPC(hipMemcpy(fft, a.snap_X + size_t(slot) * a.snap_m1, sizeof(float2) * (half + 1), hipMemcpyDeviceToHost));
+        if (power) {
+            // The slot's launch group keeps its power at first * M_max + (slot - first) * M.
+            const brp_hip::Group* grp = nullptr;
+            for (const auto& g : h->groups[0])
+                if (slot >= g.first && slot < g.first + g.count) grp = &g;
+            require(grp != nullptr, "snapshot slot outside executed batch");
+            const size_t at = size_t(grp->first) * h->m_max + size_t(slot - grp->first) * h->series[grp->series].shape.m;
+            HIPC(hipMemcpy(power, a.P + at, sizeof(float) * half, hipMemcpyDeviceToHost));
+        }
+        if (summed)  // level h of the slot at h * snap_f (the largest F of the handle)
+            for (size_t lv = 0; lv < 5; ++lv)
+                HIPC(hipMemcpy(summed + lv * p.fundamental, a.snap_summed + (size_t(slot) * 5 + lv) * a.snap_f,
+                               sizeof(float) * p.fundamental, hipMemcpyDeviceToHost));
     });
 }
 
@@ -633,9 +761,13 @@ size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap) {
     } else {
         s << "no HIP device";
     }
-    if (h && h->opt.engine == BRP_ENGINE_FUSED)
-        s << "; engine=fused N=" << h->shape.n << " C=" << h->shape.c << " R=" << h->shape.r << " S1=" << h->shape.s1
-          << " S2=" << h->shape.s2 << " group=" << h->group << " lanes=" << h->lanes;
+    if (h && h->opt.engine == BRP_ENGINE_FUSED) {
+        s << "; engine=fused";
+        for (const auto& ser : h->series)
+            s << " N=" << ser.n << " C=" << ser.shape.c << " R=" << ser.shape.r << " S1=" << ser.shape.s1
+              << " S2=" << ser.shape.s2 << ";";
+        s << " group=" << h->group << " lanes=" << h->lanes;
+    }
     else if (h)
         s << "; engine=plain N=" << h->base.n;
     const std::string text = s.str();
@@ -771,6 +903,93 @@ int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps) {
     });
 }
 
+int brp_hip_add_series(brp_hip* h, const brp_params* p, const float* input, size_t count) {
+    return guarded([&] {
+        require(h && p && input, "null argument");
+        require(h->opt.engine == BRP_ENGINE_FUSED, "more than one series needs the fused engine");
+        require(count == p->unpadded, "input length mismatch");
+        require(p->n >= 64 && p->steps && p->steps <= p->unpadded && p->unpadded <= p->n, "invalid template");
+        HIPC(hipStreamSynchronize(h->stream));
+        float* x = h->alloc<float>(size_t(p->unpadded) + 4);
+        HIPC(hipMemcpy(x, input, sizeof(float) * p->unpadded, hipMemcpyHostToDevice));
+        HIPC(hipMemset(x + p->unpadded, 0, 4 * sizeof(float)));
+        add_fused_series(*h, *p, x);
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// The bank loop of src/workunit_main.cpp:77-136 over prepared templates, each retained
+// into the toplist of its series (tops[series]).  Batches of options.batch templates are
+// launched with the templates of one series next to each other (so launch groups are
+// whole), thresholds frozen at each batch start, and retained in template order.
+void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit* orbits, brp_toplist* const* tops,
+               size_t ntops, uint32_t* cand_count, brp_params* templates_out, double* gpu_ms) {
+    const size_t count = base.size();
+    std::vector<uint32_t> ser(count, 0);
+    for (size_t i = 0; i < count; ++i) {
+        check_template(h, base[i]);
+        if (h.opt.engine == BRP_ENGINE_FUSED) ser[i] = series_of(h, base[i]);
+        require(ser[i] < ntops && tops[ser[i]], "no toplist for a template's series");
+    }
+    const size_t batch = h.opt.batch;
+    const size_t batches = (count + batch - 1) / batch;
+    // The fused engine keeps two batches in flight: batch b + 1 is enqueued with the
+    // thresholds of the toplists before batch b is retained.  Those are never higher
+    // than the thresholds batch b + 1 would see one batch at a time, so its candidates
+    // are a superset, and retain() with each template's current thresholds keeps
+    // exactly the same rows.  The plain engine runs batch by batch, as the reference.
+    const bool overlap = h.opt.engine == BRP_ENGINE_FUSED && !h.opt.snapshot;
+    std::vector<uint32_t> order[2];  // order[half][slot] = template index within the batch
+    auto enqueue = [&](size_t b) {
+        const int half = overlap ? int(b & 1) : 0;
+        const size_t first = b * batch, n = std::min(batch, count - first);
+        auto& ord = order[half];
+        ord.resize(n);
+        for (size_t k = 0; k < n; ++k) ord[k] = uint32_t(k);
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](uint32_t x, uint32_t y) { return ser[first + x] < ser[first + y]; });
+        h.templates[half].resize(n);
+        for (size_t k = 0; k < n; ++k) {
+            h.templates[half][k] = base[first + ord[k]];
+            brp_toplist_thresholds(tops[ser[first + ord[k]]], &h.templates[half][k]);
+        }
+        launch(h, half);
+    };
+    double total_gpu = 0;
+    std::vector<brp_candidate> cand;
+    std::vector<uint32_t> slot_of;
+    if (batches) enqueue(0);
+    for (size_t b = 0; b < batches; ++b) {
+        const int half = overlap ? int(b & 1) : 0;
+        if (overlap && b + 1 < batches) enqueue(b + 1);
+        finish(h, half);
+        for (int s = 0; s < 4; ++s) total_gpu += h.kernel_ms[half][s];
+        const size_t first = b * batch, n = h.templates[half].size();
+        slot_of.assign(n, 0);
+        for (size_t k = 0; k < n; ++k) slot_of[order[half][k]] = uint32_t(k);
+        for (size_t k = 0; k < n; ++k) {
+            const size_t i = first + k;
+            brp_params now = base[i];
+            brp_toplist* top = tops[ser[i]];
+            brp_toplist_thresholds(top, &now);  // the toplist as of this template
+            read_slot(h, half, slot_of[k], cand);
+            if (int rc = brp_toplist_retain(top, cand.data(), cand.size(), &orbits[i], &now))
+                throw Failure(rc, brp_last_error());
+            if (templates_out) templates_out[i] = now;
+            if (cand_count) cand_count[i] = uint32_t(cand.size());
+        }
+        if (!overlap && b + 1 < batches) enqueue(b + 1);
+    }
+    if (gpu_ms) *gpu_ms = total_gpu;
+}
+
+}  // namespace
+
+extern "C" {
+
 int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint32_t samples, double sample_us,
                         double f0, double false_alarm, uint32_t window, brp_toplist* top, brp_params* templates_out,
                         uint32_t* cand_count, double* gpu_ms) {
@@ -781,43 +1000,17 @@ int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint3
         for (size_t i = 0; i < count; ++i)
             if (int rc = brp_search_parameters(h->base.n, samples, sample_us, &orbits[i], f0, false_alarm, window, &base[i]))
                 throw Failure(rc, brp_last_error());
-        for (const auto& p : base) check_template(h->base, p);
-        const size_t batch = h->opt.batch;
-        const size_t batches = (count + batch - 1) / batch;
-        // The fused engine keeps two batches in flight: batch b + 1 is enqueued with the
-        // thresholds of the toplist before batch b is retained.  Those are never higher
-        // than the thresholds batch b + 1 would see one batch at a time, so its candidates
-        // are a superset, and retain() with each template's current thresholds keeps
-        // exactly the same rows.  The plain engine runs batch by batch, as the reference.
-        const bool overlap = h->opt.engine == BRP_ENGINE_FUSED && !h->opt.snapshot;
-        auto enqueue = [&](size_t b) {
-            const int half = overlap ? int(b & 1) : 0;
-            const size_t first = b * batch, n = std::min(batch, count - first);
-            h->templates[half].assign(base.begin() + first, base.begin() + first + n);
-            for (auto& p : h->templates[half]) brp_toplist_thresholds(top, &p);
-            launch(*h, half);
-        };
-        double total_gpu = 0;
-        std::vector<brp_candidate> cand;
-        if (batches) enqueue(0);
-        for (size_t b = 0; b < batches; ++b) {
-            const int half = overlap ? int(b & 1) : 0;
-            if (overlap && b + 1 < batches) enqueue(b + 1);
-            finish(*h, half);
-            for (int s = 0; s < 4; ++s) total_gpu += h->kernel_ms[half][s];
-            const size_t first = b * batch, n = h->templates[half].size();
-            for (size_t k = 0; k < n; ++k) {
-                brp_params now = base[first + k];
-                brp_toplist_thresholds(top, &now);  // the toplist as of this template
-                read_slot(*h, half, uint32_t(k), cand);
-                if (int rc = brp_toplist_retain(top, cand.data(), cand.size(), &orbits[first + k], &now))
-                    throw Failure(rc, brp_last_error());
-                if (templates_out) templates_out[first + k] = now;
-                if (cand_count) cand_count[first + k] = uint32_t(cand.size());
-            }
-            if (!overlap && b + 1 < batches) enqueue(b + 1);
-        }
-        if (gpu_ms) *gpu_ms = total_gpu;
+        brp_toplist* tops[1] = {top};
+        bank_loop(*h, base, orbits, tops, 1, cand_count, templates_out, gpu_ms);
+    });
+}
+
+int brp_hip_bank_search_templates(brp_hip* h, const brp_params* templates, const brp_orbit* orbits, size_t count,
+                                  brp_toplist* const* tops, size_t ntops, uint32_t* cand_count, double* gpu_ms) {
+    return guarded([&] {
+        require(h && templates && orbits && tops, "null argument");
+        std::vector<brp_params> base(templates, templates + count);
+        bank_loop(*h, base, orbits, tops, ntops, cand_count, nullptr, gpu_ms);
     });
 }
 

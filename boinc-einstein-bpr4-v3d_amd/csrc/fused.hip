@@ -222,8 +222,8 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
                 acc += double(pair[h]);
                 if constexpr (SNAP) {
                     if (in_series) {
-                        a.snap_resampled[size_t(slot) * sh.n + i] = pair[h];
-                        a.snap_index[size_t(slot) * sh.n + i] = jj;
+                        a.snap_resampled[size_t(slot) * a.snap_n + i] = pair[h];
+                        a.snap_index[size_t(slot) * a.snap_n + i] = jj;
                     }
                 }
             }
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     __syncthreads();
 
     float* P = a.P + size_t(g) * M;
-    float2* snapX = SNAP ? a.snap_X + size_t(slot) * (M + 1) : nullptr;
+    float2* snapX = SNAP ? a.snap_X + size_t(slot) * a.snap_m1 : nullptr;
     const float inv_n = 1.0f / float(N);
     const uint32_t steps = p.steps;
     const float pad_len = float(N - steps);

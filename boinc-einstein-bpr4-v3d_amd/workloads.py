@@ -8,6 +8,7 @@ double) so that every run and every machine sees the same bytes.
   plain_full    N = 2^22, one matching template, random spin in [50, 500] Hz
   bank          N = 2^22 (or smaller), seeded orbit bank, bank semantics
   mixed_radix   N = 3 * 2^20, bench semantics, several templates
+  uneven        several series lengths (2^k and 3 * 2^k), one seeded bank split over them
 """
 from __future__ import annotations
 
@@ -92,3 +93,34 @@ def algorithmic_bytes(n: int, unpadded: int, harmonic: int) -> int:
     (4 U), one four-step exchange of N/2 complex64 values written and read back (8 N),
     the power below H written and read back (8 H)."""
     return 4 * unpadded + 8 * n + 8 * harmonic
+
+
+@dataclass
+class Uneven:
+    name: str
+    series: list            # [(n, x float32, spin_hz)] in registration order
+    templates: list         # bank-semantics parameter blocks, any series
+    orbits: list
+    series_of: list         # series index of each template
+
+
+def uneven(count: int = 2048, seed: int = 42, lengths=(1 << 20, 3 << 19, 1 << 22), xs=None) -> Uneven:
+    """One observation per length (seed + 10 k, its own spin frequency, the injected orbit)
+    and `count` seeded orbits split over the lengths by a seeded uniform draw; the first
+    template of each length is the injected orbit.  Bank semantics on each series.  xs
+    (optional) replaces the generated float series (for example by preprocessed ones)."""
+    series = []
+    for k, n in enumerate(lengths):
+        spin = spin_for(seed + 10 * k)
+        x = _series(n, seed + 10 * k, spin) if xs is None else np.ascontiguousarray(xs[k], np.float32)
+        series.append((n, x, spin))
+    orbits = [(o.period, o.tau, o.psi) for o in brp.synthetic_bank(seed + 1, count)]
+    which = [min(int(len(lengths) * brp.uniform(seed + 7, i)), len(lengths) - 1) for i in range(count)]
+    seen = set()
+    for i, s in enumerate(which):
+        if s not in seen:
+            orbits[i] = INJECTED
+            seen.add(s)
+    templates = [brp.search_parameters(lengths[s], lengths[s], SAMPLE_US, o) for s, o in zip(which, orbits)]
+    return Uneven(f"uneven {count} over N={list(lengths)}", series, templates, orbits, which)
+

@@ -198,6 +198,24 @@ int brp_hip_autotune(const brp_params* templates, size_t count, const float* inp
  * GB/s; used to calibrate hardware byte counters against a transfer of known size. */
 int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps);
 
+/* Uneven batches (the batched extension of set_templates, src/vulkan.cpp:289-295, which
+ * lifts the reference's fixed-dimension rule): registers one more observation of
+ * p->unpadded samples, padded to p->n (2^k or 3 2^k), on a fused-engine handle.  A
+ * template then names its series by (n, unpadded); templates of every registered series
+ * can share one batch, one launch schedule and one graph, each with its own F and H.
+ * The capacity and arithmetic flags stay those of the handle.  Option overrides of the
+ * four-step split apply where they fit the length, else its default shape is used. */
+int brp_hip_add_series(brp_hip* h, const brp_params* p, const float* input, size_t count);
+/* Bank search over prepared templates (brp_search_parameters or brp_parameters results,
+ * any registered series), the bank loop of src/workunit_main.cpp:77-136 with one toplist
+ * per series: tops[i] collects the templates of series i (registration order, the
+ * constructor's series first), ntops >= the number of series named.  Thresholds are
+ * frozen per batch and retain() replayed in template order, so each toplist equals the
+ * one-template-at-a-time result for its series.  cand_count (may be NULL) receives the
+ * candidate count of each template. */
+int brp_hip_bank_search_templates(brp_hip* h, const brp_params* templates, const brp_orbit* orbits, size_t count,
+                                  brp_toplist* const* tops, size_t ntops, uint32_t* cand_count, double* gpu_ms);
+
 /* Bank search: the brp_workunit bank loop (src/workunit_main.cpp:77-136) for a bank of
  * `count` orbits against the handle's series, in batches of options.batch templates,
  * with thresholds frozen at each batch start and Toplist::retain replayed in template

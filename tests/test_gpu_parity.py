@@ -264,6 +264,51 @@ def test_bank_toplist_identical(brp, oracle, wl):
     assert sum(r.harmonics > 0 for r in ref.state) > 0
 
 
+def test_fused_uneven_batch_parity(brp, oracle, wl):
+    """Templates of three series lengths (2^k and 3 * 2^k) in one batch: every slot meets
+    the full parity bar against the oracle on its own series."""
+    u = wl.uneven(count=9, lengths=(1 << 17, 3 << 16, 1 << 18))
+    say(f"[gpu] fused uneven batch of 9 over N={[s[0] for s in u.series]}, snapshots")
+    xs = {n: x for n, x, _ in u.series}
+    first = u.templates[0]
+    with brp.GPU(first, xs[first.n], 256, 0, 9, packed=True, engine=brp.ENGINE_FUSED, snapshot=True) as g:
+        for n, x, _ in u.series:
+            if n != first.n:
+                g.add_series(brp.search_parameters(n, n, wl.SAMPLE_US, wl.INJECTED), x)
+        g.set_templates(u.templates)
+        g.run()
+        for s, p in enumerate(u.templates):
+            rep = parity.check_template(g.snapshot(s), oracle, xs[p.n], p, f"uneven/slot{s}/N={p.n}")
+            say(rep)
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_bank_uneven_toplists(brp, oracle, wl, lanes):
+    """A bank split over three lengths in one launch schedule (mixed batches, graph): each
+    series' toplist equals the oracle's one-template-at-a-time loop over its templates."""
+    u = wl.uneven(count=40, lengths=(1 << 18, 3 << 17, 1 << 19))
+    say(f"[gpu] bank of 40 over N={[s[0] for s in u.series]}, batch 16, group 4, lanes {lanes}")
+    first = u.templates[0]
+    xs = {n: x for n, x, _ in u.series}
+    order = [n for n, _, _ in u.series]
+    reg = [first.n] + [n for n in order if n != first.n]  # registration order = toplist order
+    tops = [brp.Toplist(100) for _ in reg]
+    with brp.GPU(first, xs[first.n], 256, 0, 16, packed=True, engine=brp.ENGINE_FUSED, graph=True, group=4,
+                 lanes=lanes) as g:
+        for n in reg[1:]:
+            g.add_series(brp.search_parameters(n, n, wl.SAMPLE_US, wl.INJECTED), xs[n])
+        counts, _ = g.bank_search_templates(u.templates, u.orbits, tops)
+    assert counts.sum() > 0
+    for k, n in enumerate(reg):
+        sel = [i for i, p in enumerate(u.templates) if p.n == n]
+        ref = _oracle_bank(oracle, xs[n], [u.templates[i] for i in sel], [u.orbits[i] for i in sel])
+        rows = [(r.bin, r.harmonics, r.period) for r in tops[k].table()]
+        assert rows == [(r.bin, r.harmonics, r.period) for r in ref.state], n
+        pw = np.array([r.power for r in tops[k].table()])
+        assert np.allclose(pw, np.array([r.power for r in ref.state]), rtol=1e-5, atol=0), n
+        assert [r.bin for r in tops[k].final()] == [r.bin for r in ref.final()], n
+
+
 def test_fp64_phase_variant(brp, oracle, wl):
     """The FP64-phase variant rounds the exact orbit model; its index differences from the
     FP32 LUT rule (the parity mode) are counted and reported, and its spectrum is checked
@@ -321,13 +366,24 @@ def test_out_of_range_index_flagged(brp, wl):
 
 
 def test_template_dimension_change_rejected(brp, wl):
+    """The plain engine keeps the reference's fixed-dimension rule; the fused engine lifts
+    it for F and H and for registered series, and still rejects an unregistered length,
+    a capacity change and an oversized batch."""
     w = wl.plain_small()
     p = w.templates[0]
     q = p.copy()
     q.fundamental -= 1
+    other = brp.parameters(1 << 15, 1 << 15, p.tau, 3600.0, p.psi, p.dt)
+    cap = p.copy()
+    cap.capacity -= 1
     say("[gpu] set_templates rejects a dimension change")
-    with brp.GPU(p, w.x, 256, 0, 2, packed=True, engine=brp.ENGINE_FUSED) as g:
+    with brp.GPU(p, w.x, 256, 0, 2, stockham=True, packed=True, engine=brp.ENGINE_PLAIN) as g:
         with pytest.raises(RuntimeError, match="dimensions changed"):
             g.set_templates([p, q])
+    with brp.GPU(p, w.x, 256, 0, 2, packed=True, engine=brp.ENGINE_FUSED) as g:
+        g.set_templates([p, q])
+        for bad in (other, cap):
+            with pytest.raises(RuntimeError, match="dimensions changed"):
+                g.set_templates([p, bad])
         with pytest.raises(RuntimeError, match="batch exceeds"):
             g.set_templates([p, p, p])
