@@ -78,6 +78,8 @@ def main() -> int:
     ap.add_argument("--plain-templates", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--uneven-templates", type=int, default=2048,
+                    help="uneven workunit leg: templates over N = 2^20, 3*2^19, 2^22 (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the bank is design-only",
@@ -202,6 +204,44 @@ def main() -> int:
                    "identical": ra == rb and bool(np.allclose(pa, pb, rtol=1e-5, atol=0)) and fa == fb}
         log(f"[bench] top-1000: {top1000}")
 
+    # ------------------------------------------------------- uneven workunit --
+    # 4-bit DD blocks of three lengths (quantized N(7.5, 2^2) noise, a pulsar, 10 RFI
+    # bins at 50 sigma) decoded, whitened and RFI-cleaned on the device, then one bank
+    # split over the three series in one launch schedule, top-1000 per series.
+    uneven = None
+    if args.uneven_templates > 0:
+        lengths = (1 << 20, 3 << 19, 1 << 22)
+        xs, rfi_total, prep_ms = [], 0, 0.0
+        for k, n_s in enumerate(lengths):
+            block, _ = brp.synthetic_dd(n_s, workloads.DT, 42 + 10 * k, 7.5, 2.0, 4, workloads.spin_for(42 + 10 * k),
+                                        0.05, 0.1, workloads.INJECTED, n_s, 10, 50.0)
+            tp = time.perf_counter()
+            x_s, _, rfi = brp.preprocess(block, n_s, 4, 1.0, n_s, 1000, 30.0)
+            prep_ms += (time.perf_counter() - tp) * 1e3
+            xs.append(x_s)
+            rfi_total += rfi
+        u = workloads.uneven(count=args.uneven_templates, lengths=lengths, xs=xs)
+        first = u.templates[0]
+        byn = {n_s: x_s for n_s, x_s, _ in u.series}
+        reg = [first.n] + [n_s for n_s in lengths if n_s != first.n]
+        with brp.GPU(first, byn[first.n], 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
+                     graph=not args.no_graph, group=args.group, split_c=args.split_c, split_s1=args.split_s1,
+                     split_s2=args.split_s2) as gw:
+            for n_s in reg[1:]:
+                gw.add_series(brp.search_parameters(n_s, n_s, workloads.SAMPLE_US, workloads.INJECTED), byn[n_s])
+            gw.bank_search_templates(u.templates, u.orbits, [brp.Toplist(1000) for _ in reg])  # warm-up
+            tw0 = time.perf_counter()
+            for _ in range(args.steps):
+                tops_u = [brp.Toplist(1000) for _ in reg]
+                gw.bank_search_templates(u.templates, u.orbits, tops_u)
+            uneven_tps = len(u.templates) * args.steps / (time.perf_counter() - tw0)
+        uneven = {"value": round(uneven_tps, 3), "unit": "templates/s", "templates": len(u.templates),
+                  "lengths": list(lengths), "per_length": [u.series_of.count(k) for k in range(len(lengths))],
+                  "preprocess_ms": round(prep_ms, 2), "rfi_replaced": rfi_total,
+                  "top1000_rows": {str(n_s): sum(1 for r in tp_.table() if r.power > 0) for n_s, tp_ in zip(reg, tops_u)},
+                  "final_rows": {str(n_s): len(tp_.final()) for n_s, tp_ in zip(reg, tops_u)}}
+        log(f"[bench] uneven workunit: {uneven}")
+
     # --------------------------------------------------------------- roofline --
     B = workloads.algorithmic_bytes(n, U, H)
     # Per-kernel algorithmic bytes per template: pass 1 reads the series and writes the
@@ -276,6 +316,7 @@ def main() -> int:
         "batched_over_plain": round(templates_per_s / plain_tps, 3) if plain_tps else None,
         "toplist_identical": identical,
         "top1000": top1000,
+        "uneven_workunit": uneven,
         "power_error": power_error,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,

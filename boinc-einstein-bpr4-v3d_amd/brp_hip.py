@@ -107,6 +107,9 @@ def load() -> ctypes.CDLL:
         "brp_write_candidates": (i32, [ctypes.c_char_p, P(BankRow), sz, f64]),
         "brp_synthetic_series": (i32, [u32, f64, ctypes.c_uint64, f64, f64, f64, f64, f64, P(Orbit), fp]),
         "brp_synthetic_bank": (i32, [ctypes.c_uint64, u32, f64, f64, f64, P(Orbit)]),
+        "brp_synthetic_dd": (i32, [u32, f64, ctypes.c_uint64, f64, f64, u32, f64, f64, f64, P(Orbit), u32, u32, f64,
+                                   P(ctypes.c_uint8), P(u32)]),
+        "brp_hip_preprocess": (i32, [P(ctypes.c_uint8), sz, u32, f64, u32, u32, f32, fp, fp, P(u32)]),
         "brp_uniform": (f64, [ctypes.c_uint64, ctypes.c_uint64]),
         "brp_hip_create": (i32, [P(Params), fp, sz, P(Options), P(ctypes.c_void_p)]),
         "brp_hip_destroy": (i32, [ctypes.c_void_p]),
@@ -181,6 +184,33 @@ def synthetic_series(count: int, dt: float, seed: int, mean: float = 0.0, sigma:
     _check(load().brp_synthetic_series(count, dt, seed, mean, sigma, spin_hz, duty, amplitude,
                                        ctypes.byref(o) if o is not None else None, _fptr(out)))
     return out
+
+
+def synthetic_dd(samples: int, dt: float, seed: int, mean: float = 7.5, sigma: float = 2.0, bits: int = 4,
+                 spin_hz: float = 0.0, duty: float = 0.05, amplitude: float = 0.0, orbit=None, n: int = 0,
+                 rfi_count: int = 0, rfi_sigma: float = 50.0):
+    """DD sample block (4-bit packed, high nibble first, or signed 8-bit); returns (bytes, rfi bins)."""
+    n = n or samples
+    out = np.zeros(samples // 2 if bits == 4 else samples, dtype=np.uint8)
+    bins = np.zeros(max(rfi_count, 1), dtype=np.uint32)
+    o = None if orbit is None else (orbit if isinstance(orbit, Orbit) else Orbit(*orbit))
+    _check(load().brp_synthetic_dd(samples, dt, seed, mean, sigma, bits, spin_hz, duty, amplitude,
+                                   ctypes.byref(o) if o is not None else None, n, rfi_count, rfi_sigma,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                   bins.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+    return out, bins[:rfi_count].copy()
+
+
+def preprocess(block: np.ndarray, samples: int, bits: int, scale: float, n: int, window: int = 1000,
+               rfi_ratio: float = 30.0):
+    """src/workload.cpp:29-95 on the device: returns (preprocessed, decoded, rfi bins replaced)."""
+    b = np.ascontiguousarray(block, dtype=np.uint8)
+    out = np.empty(samples, dtype=np.float32)
+    dec = np.empty(samples, dtype=np.float32)
+    rfi = ctypes.c_uint32()
+    _check(load().brp_hip_preprocess(b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), samples, bits, scale, n,
+                                     window, rfi_ratio, _fptr(out), _fptr(dec), ctypes.byref(rfi)))
+    return out, dec, rfi.value
 
 
 def synthetic_bank(seed: int, count: int, pmin: float = 660.0, pmax: float = 10000.0, tau_max: float = 0.3):

@@ -88,6 +88,13 @@ void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a);
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles);
 // bytes / 16 uint4 copies from src to dst (a multiple of 16 bytes).
 void copy16(hipStream_t st, const void* src, void* dst, size_t bytes);
+// ------------------------------------------------------------ preprocess --
+// Decode + whitening + RFI replacement of one observation (preprocess.hip); throws
+// std::runtime_error on an invalid median.  decoded (may be null) receives the decoded
+// samples, rfi_bins (may be null) the number of replaced bins.
+void preprocess_observation(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                            uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins);
+
 // Copies {count, flags} and the first `keep` candidates of each of `nslots` slots into a
 // dense staging area of (keep + 1) x 16 bytes per slot.
 void pack_results(hipStream_t st, const uint32_t* results, uint32_t slot_words, uint32_t nslots, uint32_t keep,

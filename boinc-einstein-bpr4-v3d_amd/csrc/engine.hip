@@ -903,6 +903,24 @@ int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps) {
     });
 }
 
+int brp_hip_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                       uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins) {
+    return guarded([&] {
+        require(bytes && out, "null argument");
+        require(bits == 4 || bits == 8, "unsupported DD sample format");
+        require(!(bits == 4 && (samples & 1)), "odd packed DD length");
+        require(samples && samples <= n && std::isfinite(scale) && scale > 0, "invalid DD metadata");
+        require(n >= 64 && n <= 3u * (1u << 22) && n % 2 == 0 && (pow2(n) || (n % 3 == 0 && pow2(n / 3))),
+                "observation exceeds supported length");
+        require(window && !(window & 1) && window <= n / 2 && window <= 3072, "unsupported whitening window");
+        require(std::isfinite(rfi_ratio) && rfi_ratio >= 0, "invalid RFI ratio");
+        int ndev = 0;
+        HIPC(hipGetDeviceCount(&ndev));
+        require(ndev > 0, "no HIP device");
+        brp_engine::preprocess_observation(bytes, samples, bits, scale, n, window, rfi_ratio, out, decoded, rfi_bins);
+    });
+}
+
 int brp_hip_add_series(brp_hip* h, const brp_params* p, const float* input, size_t count) {
     return guarded([&] {
         require(h && p && input, "null argument");

@@ -328,6 +328,39 @@ int brp_synthetic_series(uint32_t count, double dt, uint64_t seed, double mean, 
     return BRP_OK;
 }
 
+int brp_synthetic_dd(uint32_t samples, double dt, uint64_t seed, double mean, double sigma, uint32_t bits,
+                     double spin_hz, double duty, double amplitude, const brp_orbit* orbit, uint32_t n,
+                     uint32_t rfi_count, double rfi_sigma, uint8_t* bytes, uint32_t* rfi_bins) {
+    if (!bytes || (bits != 4 && bits != 8) || (bits == 4 && (samples & 1)) || samples > n || n < 64)
+        return brp::fail(BRP_E_INVALID, "invalid synthetic DD block");
+    std::vector<float> x(samples);
+    if (int rc = brp_synthetic_series(samples, dt, seed, mean, sigma, spin_hz, duty, amplitude, orbit, x.data()))
+        return rc;
+    // Narrowband RFI: a cosine at bin k of the n-point spectrum whose power there is
+    // rfi_sigma^2 times the mean noise power per bin (amplitude 2 rfi_sigma sigma / sqrt(n)).
+    const uint64_t rseed = seed ^ 0x5bf03635d1f6a0c9ull;
+    for (uint32_t r = 0; r < rfi_count; ++r) {
+        const uint32_t lo = n / 64, hi = n / 2 - n / 64;
+        const uint32_t k = lo + uint32_t(brp_uniform(rseed, 2ull * r) * double(hi - lo));
+        const double phase = kTwoPi * brp_uniform(rseed, 2ull * r + 1);
+        const double a = 2.0 * rfi_sigma * sigma / std::sqrt(double(n));
+        if (rfi_bins) rfi_bins[r] = k;
+        for (uint32_t j = 0; j < samples; ++j)
+            x[j] = float(double(x[j]) + a * std::cos(kTwoPi * double(uint64_t(k) * j % n) / double(n) + phase));
+    }
+    for (uint32_t j = 0; j < samples; ++j) {
+        const double q = std::nearbyint(double(x[j]));
+        if (bits == 4) {
+            const uint32_t v = uint32_t(std::min(15.0, std::max(0.0, q)));
+            if (j & 1) bytes[j / 2] = uint8_t(bytes[j / 2] | v);
+            else bytes[j / 2] = uint8_t(v << 4);  // high nibble first (src/workload.cpp:53)
+        } else {
+            bytes[j] = uint8_t(int8_t(std::min(127.0, std::max(-128.0, q))));
+        }
+    }
+    return BRP_OK;
+}
+
 int brp_synthetic_bank(uint64_t seed, uint32_t count, double pmin, double pmax, double tau_max,
                        brp_orbit* out) {
     if (!out || !(pmin > 0) || !(pmax >= pmin) || !(tau_max >= 0))

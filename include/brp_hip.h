@@ -140,12 +140,37 @@ int brp_write_candidates(const char* path, const brp_bank_row* rows, size_t coun
 int brp_synthetic_series(uint32_t count, double dt, uint64_t seed, double mean, double sigma,
                          double spin_hz, double duty, double amplitude, const brp_orbit* orbit,
                          float* out);
+/* A DD sample block (the bytes after the header, src/workload.cpp:45-53): the series of
+ * brp_synthetic_series (mean, sigma, pulse train) plus rfi_count narrowband cosines at
+ * seeded bins k in [n/64, n/2 - n/64) of the n-point spectrum with power rfi_sigma^2 times
+ * the mean noise power there, rounded to the nearest integer and clamped: bits 4 ->
+ * [0, 15], two per byte, high nibble first; bits 8 -> signed [-128, 127].  rfi_bins (may
+ * be NULL) receives the rfi_count bins. */
+int brp_synthetic_dd(uint32_t samples, double dt, uint64_t seed, double mean, double sigma, uint32_t bits,
+                     double spin_hz, double duty, double amplitude, const brp_orbit* orbit, uint32_t n,
+                     uint32_t rfi_count, double rfi_sigma, uint8_t* bytes, uint32_t* rfi_bins);
 /* Orbits drawn from a seed: P log-uniform in [pmin, pmax], tau uniform in [0, tau_max],
  * psi uniform in [0, 2 pi). */
 int brp_synthetic_bank(uint64_t seed, uint32_t count, double pmin, double pmax, double tau_max,
                        brp_orbit* out);
 /* Counter-based uniform in [0, 1) (SplitMix64 of seed and counter). */
 double brp_uniform(uint64_t seed, uint64_t counter);
+
+/* ---------------------------------------------------- preprocessing -------- */
+/* load_observation + preprocess (src/workload.cpp:29-95) on the device, for the sample
+ * block of a DD file (after its header): decode (bits = 4: packed, high nibble first;
+ * bits = 8: signed bytes; each value float(double(v) / scale)), zero-pad to n, real FFT,
+ * power float(re^2 + im^2), running median of `window` bins (even, 2 <= window <= 3072,
+ * window <= n / 2; even-window median = mean of the two middle values), whitening by
+ * sqrt(ln 2 / median) with the bins outside [window/2, n/2 + 1 - window/2) zeroed, and
+ * inverse FFT scaled by 1/sqrt(n).  RFI (new behaviour; the reference zaps a listed range
+ * with GSL noise): a bin whose power exceeds rfi_ratio x its running median (rfi_ratio 0 =
+ * off) is scaled to the median first.  out: `samples` floats (the reference's
+ * Observation::preprocessed); decoded (may be NULL): `samples` floats; rfi_bins (may be
+ * NULL): bins replaced.  Errors: invalid arguments, or a non-positive median
+ * ("invalid whitening median", as src/workload.cpp:68). */
+int brp_hip_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                       uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins);
 
 /* ------------------------------------------------------------ device -------- */
 

@@ -478,6 +478,61 @@ static void cpu_template(const float* input, size_t count, const oracle_params& 
     oracle_select(sum.data(), &p, cand.data(), cand.size());
 }
 
+// src/workload.cpp:29-95 (decode and preprocess), with the running median and RFI rule
+// stated in oracle.h (the historical rngmed and the GSL zap noise are not available).
+int oracle_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                      uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins) {
+    if (!bytes || !out || (bits != 4 && bits != 8) || samples > n || n % 2 || !window || (window & 1) ||
+        window > n / 2)
+        return fail("invalid preprocessing arguments");
+    std::vector<double> time(n, 0.0);
+    for (size_t i = 0; i < samples; ++i) {
+        double v;
+        if (bits == 4) v = double((i & 1) ? (bytes[i / 2] & 15) : (bytes[i / 2] >> 4));
+        else v = double(int8_t(bytes[i]));
+        const float f = float(v / scale);  // src/workload.cpp:52-53
+        time[i] = f;
+        if (decoded) decoded[i] = f;
+    }
+    const size_t m = n / 2, bins = m + 1;
+    std::vector<double> spec(2 * bins);
+    rfft<double>(time.data(), n, spec.data());
+    std::vector<float> power(bins, 0.0f);
+    for (size_t k = 1; k < bins; ++k) power[k] = float(spec[2 * k] * spec[2 * k] + spec[2 * k + 1] * spec[2 * k + 1]);
+    const size_t nmed = bins - window + 1;
+    std::vector<float> median(nmed), win(window);
+    for (size_t i = 0; i < nmed; ++i) {
+        std::copy(power.begin() + i, power.begin() + i + window, win.begin());
+        std::nth_element(win.begin(), win.begin() + window / 2, win.end());
+        const float hi = win[window / 2];
+        const float lo = *std::max_element(win.begin(), win.begin() + window / 2);
+        median[i] = float((double(lo) + double(hi)) * 0.5);
+        if (!(median[i] > 0) || !std::isfinite(median[i])) return fail("invalid whitening median");
+    }
+    uint32_t rfi = 0;
+    std::vector<std::complex<double>> full(n, {0.0, 0.0});
+    for (size_t k = window / 2; k < bins - window / 2; ++k) {
+        const float med = median[k - window / 2];
+        std::complex<double> x(spec[2 * k], spec[2 * k + 1]);
+        if (rfi_ratio > 0 && power[k] > rfi_ratio * med) {
+            x *= std::sqrt(double(med) / double(power[k]));
+            ++rfi;
+        }
+        x *= double(float(std::sqrt(0.69314718055994530942 / double(med))));
+        full[k] = x;
+        if (k < m) full[n - k] = std::conj(x);
+    }
+    // Unnormalised inverse over the full Hermitian spectrum: y_j = Re FFT(conj X)_j.
+    for (auto& c : full) c = std::conj(c);
+    Plan<double> plan(n);
+    std::vector<std::complex<double>> scratch(n);
+    const std::complex<double>* y = plan.run(full.data(), scratch.data());
+    const double inv = 1.0 / std::sqrt(double(n));
+    for (size_t j = 0; j < samples; ++j) out[j] = float(y[j].real() * inv);
+    if (rfi_bins) *rfi_bins = rfi;
+    return 0;
+}
+
 double oracle_benchmark_cpu(const float* input, size_t count, const oracle_params* tpl,
                             size_t ntpl, uint32_t threads, uint32_t reps) {
     if (!threads) threads = 1;

@@ -97,6 +97,17 @@ size_t oracle_toplist_final(const oracle_bank_row* state, uint32_t rows, oracle_
 double oracle_benchmark_cpu(const float* input, size_t count, const oracle_params* templates,
                             size_t ntemplates, uint32_t threads, uint32_t reps);
 
+/* ---- observation preprocessing (src/workload.cpp:29-95) ---- */
+/* Decode (4-bit high nibble first, or signed 8-bit; float(double(v) / scale)),
+ * zero-pad to n, FP64 real FFT, power float(re^2 + im^2), running median of `window` bins
+ * (even window: mean of the two middle values), optional RFI replacement (power above
+ * rfi_ratio x median scaled down to the median), whitening by float(sqrt(ln 2 / median)) on
+ * [window/2, n/2 + 1 - window/2) with every other bin zero, FP64 inverse FFT over the full
+ * Hermitian spectrum, divided by sqrt(n).  out: samples floats; decoded (may be NULL):
+ * samples floats; rfi_bins (may be NULL): replaced bins. */
+int oracle_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                      uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins);
+
 /* The 65-entry (sin, cos) table used by the LUT sine, as 130 interleaved floats. */
 void oracle_lut_table(float* out);
 

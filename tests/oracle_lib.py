@@ -68,6 +68,8 @@ def load() -> ctypes.CDLL:
         "oracle_resampling_steps": (i32, [P(OParams)]),
         "oracle_parameters": (i32, [u32, u32, f32, f32, f32, f32, P(OParams)]),
         "oracle_resample": (i32, [fp, sz, P(OParams), fp, P(ctypes.c_int32)]),
+        "oracle_preprocess": (i32, [P(ctypes.c_uint8), sz, u32, ctypes.c_double, u32, u32, ctypes.c_float, fp, fp,
+                                    P(u32)]),
         "oracle_rfft_f64": (i32, [dp, sz, dp]),
         "oracle_rfft_f32": (i32, [fp, sz, fp]),
         "oracle_power": (i32, [fp, sz, fp, dp]),
@@ -223,3 +225,16 @@ def benchmark_cpu(x: np.ndarray, templates, threads: int, reps: int) -> float:
     arr = (OParams * len(templates))(*[as_oparams(t) for t in templates])
     x = np.ascontiguousarray(x, np.float32)
     return load().oracle_benchmark_cpu(_fp(x), x.size, arr, len(templates), threads, reps)
+
+
+def preprocess(block: np.ndarray, samples: int, bits: int, scale: float, n: int, window: int = 1000,
+               rfi_ratio: float = 30.0):
+    """src/workload.cpp:29-95 restated in FP64: (preprocessed, decoded, rfi bins replaced)."""
+    b = np.ascontiguousarray(block, dtype=np.uint8)
+    out = np.empty(samples, dtype=np.float32)
+    dec = np.empty(samples, dtype=np.float32)
+    rfi = ctypes.c_uint32()
+    _check(load().oracle_preprocess(b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), samples, bits, scale, n,
+                                    window, rfi_ratio, _fp(out), _fp(dec), ctypes.byref(rfi)))
+    return out, dec, rfi.value
+

@@ -309,6 +309,54 @@ def test_bank_uneven_toplists(brp, oracle, wl, lanes):
         assert [r.bin for r in tops[k].final()] == [r.bin for r in ref.final()], n
 
 
+@pytest.mark.parametrize("n,bits", [(1 << 16, 4), (3 << 15, 4), (1 << 16, 8)])
+def test_preprocess_parity(brp, oracle, n, bits):
+    """Device decode + whitening + RFI replacement against the FP64 oracle: decoded
+    samples bit for bit, the same RFI bins replaced, rel-L2 <= 1e-5 on the series."""
+    mean = 7.5 if bits == 4 else 0.0
+    block, injected = brp.synthetic_dd(n, 64e-6, 11, mean, 2.0, bits, 200.0, 0.05, 0.1, (3600.0, 0.05, 1.0), n, 10,
+                                       50.0)
+    say(f"[gpu] preprocess N={n} bits={bits}")
+    out, dec, rfi = brp.preprocess(block, n, bits, 1.5, n, 1000, 30.0)
+    ref, rdec, rrfi = oracle.preprocess(block, n, bits, 1.5, n, 1000, 30.0)
+    assert np.array_equal(dec, rdec)
+    assert rfi == rrfi == 10
+    rel = float(np.linalg.norm(out.astype(np.float64) - ref) / np.linalg.norm(ref))
+    say({"rel_l2": rel, "max_abs": float(np.max(np.abs(out - ref)))})
+    assert rel <= 1e-5
+
+
+def test_uneven_workunit_pipeline(brp, oracle, wl):
+    """The uneven workunit end to end at reduced sizes: 4-bit DD blocks of three lengths
+    with pulsars and RFI, preprocessed on the device, then one mixed bank in one launch
+    schedule; each series' toplist equals the oracle loop on the same preprocessed series."""
+    lengths = (1 << 17, 3 << 16, 1 << 18)
+    xs = []
+    for k, n in enumerate(lengths):
+        spin = wl.spin_for(42 + 10 * k)
+        block, _ = brp.synthetic_dd(n, wl.DT, 42 + 10 * k, 7.5, 2.0, 4, spin, 0.05, 0.1, wl.INJECTED, n, 10, 50.0)
+        x, _, rfi = brp.preprocess(block, n, 4, 1.0, n, 1000, 30.0)
+        assert rfi == 10
+        xs.append(x)
+    u = wl.uneven(count=36, lengths=lengths, xs=xs)
+    say(f"[gpu] uneven workunit: 36 templates over N={list(lengths)} (4-bit, whitened, RFI replaced)")
+    first = u.templates[0]
+    byn = {n: x for n, x, _ in u.series}
+    reg = [first.n] + [n for n in lengths if n != first.n]
+    tops = [brp.Toplist(100) for _ in reg]
+    with brp.GPU(first, byn[first.n], 256, 0, 16, packed=True, engine=brp.ENGINE_FUSED, graph=True, group=4) as g:
+        for n in reg[1:]:
+            g.add_series(brp.search_parameters(n, n, wl.SAMPLE_US, wl.INJECTED), byn[n])
+        counts, _ = g.bank_search_templates(u.templates, u.orbits, tops)
+    assert counts.sum() > 0
+    for k, n in enumerate(reg):
+        sel = [i for i, p in enumerate(u.templates) if p.n == n]
+        ref = _oracle_bank(oracle, byn[n], [u.templates[i] for i in sel], [u.orbits[i] for i in sel])
+        assert [(r.bin, r.harmonics, r.period) for r in tops[k].table()] == \
+            [(r.bin, r.harmonics, r.period) for r in ref.state], n
+        assert [r.bin for r in tops[k].final()] == [r.bin for r in ref.final()], n
+
+
 def test_fp64_phase_variant(brp, oracle, wl):
     """The FP64-phase variant rounds the exact orbit model; its index differences from the
     FP32 LUT rule (the parity mode) are counted and reported, and its spectrum is checked

@@ -17,6 +17,8 @@ this path, so each oracle function is pinned by checks that a plausible mistake
                    gather == scatter bit-exact on random spectra
   chi^2            scipy.stats.chi2.sf / isf
   toplist          hand-built replacement / ordering / cross-level cases
+  preprocess       independent numpy restatement (numpy.fft.rfft / irfft, numpy.median of
+                   each window), decode bit for bit, injected RFI bins found and replaced
 """
 import json
 import math
@@ -300,3 +302,53 @@ def test_toplist_replacement_and_final(oracle):
     q = oracle.search_parameters(1 << 20, 1 << 20, 64.0, (3600.0, 0.05, 1.0))
     top.thresholds(q)
     assert q.threshold[0] == np.float32(21.0)
+
+
+def _numpy_preprocess(block, samples, bits, scale, n, window, rfi_ratio):
+    if bits == 4:
+        v = np.empty(samples, dtype=np.float64)
+        v[0::2] = block >> 4
+        v[1::2] = block & 15
+    else:
+        v = block.view(np.int8).astype(np.float64)
+    dec = (v / scale).astype(np.float32)
+    X = np.fft.rfft(np.concatenate([dec.astype(np.float64), np.zeros(n - samples)]))
+    P = (X.real ** 2 + X.imag ** 2).astype(np.float32)
+    P[0] = 0
+    win = np.lib.stride_tricks.sliding_window_view(P.astype(np.float64), window)
+    med = np.median(win, axis=1).astype(np.float32)
+    Y = np.zeros_like(X)
+    rfi = 0
+    for k in range(window // 2, len(X) - window // 2):
+        m = med[k - window // 2]
+        x = X[k]
+        if rfi_ratio > 0 and P[k] > np.float32(rfi_ratio) * m:
+            x = x * math.sqrt(float(m) / float(P[k]))
+            rfi += 1
+        Y[k] = x * float(np.float32(math.sqrt(math.log(2) / float(m))))
+    y = np.fft.irfft(Y, n) * n / math.sqrt(n)
+    return y[:samples], dec, rfi
+
+
+@pytest.mark.parametrize("n,bits", [(4096, 4), (3 * 2048, 4), (4096, 8)])
+def test_preprocess_matches_numpy_restatement(oracle, brp, n, bits):
+    window = 64
+    mean = 7.5 if bits == 4 else 0.0
+    block, rfi_bins = brp.synthetic_dd(n, 64e-6, 3, mean, 2.0, bits, 200.0, 0.05, 0.1, (3600.0, 0.05, 1.0), n, 3, 50.0)
+    out, dec, nrfi = oracle.preprocess(block, n, bits, 1.5, n, window, 30.0)
+    ref, dec_np, rfi_np = _numpy_preprocess(block, n, bits, 1.5, n, window, 30.0)
+    assert np.array_equal(dec, dec_np)
+    assert nrfi == rfi_np == 3
+    rms = float(np.sqrt(np.mean(ref ** 2)))
+    assert np.max(np.abs(out - ref)) <= 1e-6 * rms
+    # whitened noise: unit mean power per bin, so the series has unit variance (minus the
+    # zeroed edge bins)
+    assert 0.85 < float(np.var(out)) < 1.1
+
+
+def test_preprocess_rejects_bad_windows(oracle, brp):
+    block, _ = brp.synthetic_dd(1024, 64e-6, 1, n=1024)
+    for window in (0, 63, 1024):
+        with pytest.raises(RuntimeError):
+            oracle.preprocess(block, 1024, 4, 1.0, 1024, window, 0.0)
+
