@@ -221,7 +221,9 @@ def main() -> int:
             xs.append(x_s)
             rfi_total += rfi
         u = workloads.uneven(count=args.uneven_templates, lengths=lengths, xs=xs)
-        first = u.templates[0]
+        # The 2^22 series first: the tuned split applies to it, the others take their defaults.
+        nf = n if n in lengths else lengths[-1]
+        first = brp.search_parameters(nf, nf, workloads.SAMPLE_US, workloads.INJECTED)
         byn = {n_s: x_s for n_s, x_s, _ in u.series}
         reg = [first.n] + [n_s for n_s in lengths if n_s != first.n]
         with brp.GPU(first, byn[first.n], 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,

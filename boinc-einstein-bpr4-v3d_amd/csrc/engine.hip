@@ -940,9 +940,11 @@ int brp_hip_add_series(brp_hip* h, const brp_params* p, const float* input, size
 namespace {
 
 // The bank loop of src/workunit_main.cpp:77-136 over prepared templates, each retained
-// into the toplist of its series (tops[series]).  Batches of options.batch templates are
-// launched with the templates of one series next to each other (so launch groups are
-// whole), thresholds frozen at each batch start, and retained in template order.
+// into the toplist of its series (tops[series]).  Series are independent, so the bank
+// runs series by series (template order kept within each): batches of options.batch
+// templates are then whole launch groups of one length, except where two series meet,
+// and the launch plans repeat, so captured graphs are replayed.  Thresholds are frozen
+// at each batch start and retain() replays template order within each series.
 void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit* orbits, brp_toplist* const* tops,
                size_t ntops, uint32_t* cand_count, brp_params* templates_out, double* gpu_ms) {
     const size_t count = base.size();
@@ -952,6 +954,9 @@ void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit*
         if (h.opt.engine == BRP_ENGINE_FUSED) ser[i] = series_of(h, base[i]);
         require(ser[i] < ntops && tops[ser[i]], "no toplist for a template's series");
     }
+    std::vector<size_t> seq(count);
+    for (size_t i = 0; i < count; ++i) seq[i] = i;
+    std::stable_sort(seq.begin(), seq.end(), [&](size_t x, size_t y) { return ser[x] < ser[y]; });
     const size_t batch = h.opt.batch;
     const size_t batches = (count + batch - 1) / batch;
     // The fused engine keeps two batches in flight: batch b + 1 is enqueued with the
@@ -960,25 +965,19 @@ void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit*
     // are a superset, and retain() with each template's current thresholds keeps
     // exactly the same rows.  The plain engine runs batch by batch, as the reference.
     const bool overlap = h.opt.engine == BRP_ENGINE_FUSED && !h.opt.snapshot;
-    std::vector<uint32_t> order[2];  // order[half][slot] = template index within the batch
     auto enqueue = [&](size_t b) {
         const int half = overlap ? int(b & 1) : 0;
         const size_t first = b * batch, n = std::min(batch, count - first);
-        auto& ord = order[half];
-        ord.resize(n);
-        for (size_t k = 0; k < n; ++k) ord[k] = uint32_t(k);
-        std::stable_sort(ord.begin(), ord.end(),
-                         [&](uint32_t x, uint32_t y) { return ser[first + x] < ser[first + y]; });
         h.templates[half].resize(n);
         for (size_t k = 0; k < n; ++k) {
-            h.templates[half][k] = base[first + ord[k]];
-            brp_toplist_thresholds(tops[ser[first + ord[k]]], &h.templates[half][k]);
+            const size_t i = seq[first + k];
+            h.templates[half][k] = base[i];
+            brp_toplist_thresholds(tops[ser[i]], &h.templates[half][k]);
         }
         launch(h, half);
     };
     double total_gpu = 0;
     std::vector<brp_candidate> cand;
-    std::vector<uint32_t> slot_of;
     if (batches) enqueue(0);
     for (size_t b = 0; b < batches; ++b) {
         const int half = overlap ? int(b & 1) : 0;
@@ -986,14 +985,12 @@ void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit*
         finish(h, half);
         for (int s = 0; s < 4; ++s) total_gpu += h.kernel_ms[half][s];
         const size_t first = b * batch, n = h.templates[half].size();
-        slot_of.assign(n, 0);
-        for (size_t k = 0; k < n; ++k) slot_of[order[half][k]] = uint32_t(k);
         for (size_t k = 0; k < n; ++k) {
-            const size_t i = first + k;
+            const size_t i = seq[first + k];
             brp_params now = base[i];
             brp_toplist* top = tops[ser[i]];
             brp_toplist_thresholds(top, &now);  // the toplist as of this template
-            read_slot(h, half, slot_of[k], cand);
+            read_slot(h, half, uint32_t(k), cand);
             if (int rc = brp_toplist_retain(top, cand.data(), cand.size(), &orbits[i], &now))
                 throw Failure(rc, brp_last_error());
             if (templates_out) templates_out[i] = now;

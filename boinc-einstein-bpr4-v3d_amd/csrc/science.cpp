@@ -240,11 +240,18 @@ int brp_toplist_retain(brp_toplist* t, const brp_candidate* cand, size_t count, 
         for (uint32_t i = 0; i < rows; ++i)
             if (level[i].bin == c.bin) { slot = level[i].power < c.power ? &level[i] : nullptr; break; }
         if (!slot) continue;
-        *slot = brp_bank_row{double(c.power), double(float(orbit->period)), double(float(orbit->tau)),
-                             double(float(orbit->psi)), 0.0, 1u << c.level, c.bin};
-        // glibc's qsort (the reference's) is a stable merge sort for this size.
-        std::stable_sort(level, level + rows,
-                         [](const brp_bank_row& a, const brp_bank_row& b) { return a.power > b.power; });
+        // The reference rewrites the row and re-sorts the level with qsort (glibc's, a stable
+        // merge sort for this size).  The level is sorted before the rewrite, so the same
+        // result is one stable move of the rewritten row: out of its place, then in after
+        // every row of greater power and every row of equal power that stood before it.
+        const uint32_t place = uint32_t(slot - level);
+        const brp_bank_row row{double(c.power), double(float(orbit->period)), double(float(orbit->tau)),
+                               double(float(orbit->psi)), 0.0, 1u << c.level, c.bin};
+        std::move(level + place + 1, level + rows, level + place);  // close the gap
+        uint32_t at = 0;
+        while (at < rows - 1 && (level[at].power > row.power || (level[at].power == row.power && at < place))) ++at;
+        std::move_backward(level + at, level + rows - 1, level + rows);
+        level[at] = row;
     }
     return BRP_OK;
 }

@@ -235,9 +235,10 @@ int brp_hip_add_series(brp_hip* h, const brp_params* p, const float* input, size
  * any registered series), the bank loop of src/workunit_main.cpp:77-136 with one toplist
  * per series: tops[i] collects the templates of series i (registration order, the
  * constructor's series first), ntops >= the number of series named.  Thresholds are
- * frozen per batch and retain() replayed in template order, so each toplist equals the
- * one-template-at-a-time result for its series.  cand_count (may be NULL) receives the
- * candidate count of each template. */
+ * frozen per batch and retain() replayed in template order within each series (the bank
+ * runs series by series, so batches are whole launch groups and graphs are replayed), so
+ * each toplist equals the one-template-at-a-time result for its series.  cand_count (may
+ * be NULL) receives the candidate count of each template. */
 int brp_hip_bank_search_templates(brp_hip* h, const brp_params* templates, const brp_orbit* orbits, size_t count,
                                   brp_toplist* const* tops, size_t ntops, uint32_t* cand_count, double* gpu_ms);
 

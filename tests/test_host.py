@@ -66,10 +66,13 @@ def test_parameter_validation_messages(brp):
         brp.search_parameters(1 << 16, 1 << 16, 64.0, (100.0, 0.01, 0.1), false_alarm=1.5)
 
 
-def test_toplist_equals_oracle_on_random_streams(brp, oracle):
-    rng = np.random.default_rng(12)
+@pytest.mark.parametrize("ties", [False, True])
+def test_toplist_equals_oracle_on_random_streams(brp, oracle, ties):
+    """The incremental stable insertion equals the reference's rewrite + stable sort, also
+    with many equal powers (ties keep their order of arrival)."""
+    rng = np.random.default_rng(12 + ties)
     n = 1 << 20
-    for rows in (3, 100):
+    for rows in (3, 7, 100):
         top, otop = brp.Toplist(rows), oracle.OToplist(rows)
         for t in range(60):
             orbit = (float(rng.uniform(660, 10000)), float(rng.uniform(0, 0.3)), float(rng.uniform(0, 6.28)))
@@ -82,7 +85,10 @@ def test_toplist_equals_oracle_on_random_streams(brp, oracle):
             levels = rng.integers(0, 5, k)
             bins = rng.integers(p.window, p.fundamental, k)
             bins[: k // 3] = rng.integers(p.window, p.window + 5, k // 3)  # repeated bins
-            powers = np.float32([p.threshold[l] * rng.uniform(0.9, 3.0) for l in levels])
+            if ties:  # a few distinct levels of power, far above every threshold
+                powers = np.float32([100.0 * int(rng.integers(1, 4)) for _ in levels])
+            else:
+                powers = np.float32([p.threshold[l] * rng.uniform(0.9, 3.0) for l in levels])
             cands = np.zeros(k, dtype=brp.CANDIDATE_DTYPE)
             cands["bin"], cands["level"], cands["power"] = bins, levels, powers
             cands = np.sort(cands, order=["level", "bin"])
