@@ -51,7 +51,8 @@ def measured_traffic(kernel: str, n: int, shape: dict):
         data = json.loads(text[text.index("```json") + 7:text.rindex("```")])
     except (OSError, ValueError):
         return None, None
-    if n != 1 << 22 or shape.get("C") != data["split_c"] or shape.get("group") != data["group"]:
+    if n != 1 << 22 or any(shape.get(k) != data.get(f) for k, f in
+                           (("C", "split_c"), ("S1", "split_s1"), ("S2", "split_s2"), ("group", "group"))):
         return None, None
     for name, v in data["kernels"].items():
         if name.startswith(KERNEL_PREFIX[kernel]):
