@@ -246,4 +246,114 @@ __device__ __forceinline__ void append_candidate(uint32_t* slot, uint32_t capaci
     }
 }
 
+// ------------------------------------------- split-FP16 MFMA DFT-16 (variant) --
+// The precision variant runs a 16-point DFT stage as a real 32 x 32 GEMM on the matrix
+// cores: rows i = 2k' + c' (output bin k', re/im), columns k = 2q + c (input q, re/im),
+// F[2k'][2q] = F[2k'+1][2q+1] = cos t, F[2k'][2q+1] = sin t, F[2k'+1][2q] = -sin t with
+// t = 2 pi (k' q mod 16) / 16.  Each wave holds 64 butterflies (one per lane); one
+// v_mfma_f32_32x32x8_f16 takes 32 of them and 8 of the 32 rows of k.  Inputs and the
+// matrix are split x = hi + lo into FP16 (hi = f16(x), lo = f16(x - hi)) and the products
+// F_lo x_hi + F_hi x_lo + F_hi x_hi accumulate in FP32 (the F_lo x_lo term, ~2^-22
+// relative, is dropped).  The wave's inputs are scaled by a power of two so that the
+// largest magnitude is just below 2^14 (hi stays normal and finite; exact rescaling).
+// Lane layout of the 32x32x8 instruction (checked on the device): A and B operands hold
+// k = 4 (lane / 32) + e of row / column lane % 32, and accumulator register r holds row
+// 8 (r / 4) + 4 (lane / 32) + r % 4 of column lane % 32.  v_permlane32_swap moves the
+// halves between lane j and lane j + 32.
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct Dft16Mfma {
+    h4 ahi[4], alo[4];  // this lane's A operand of each k block, hi and lo parts
+};
+
+__device__ __forceinline__ float dft16_entry(int i, int k) {
+    constexpr float kCos[16] = {1.0f, kC16, kS8, kS16, 0.0f, -kS16, -kS8, -kC16,
+                                -1.0f, -kC16, -kS8, -kS16, 0.0f, kS16, kS8, kC16};
+    const int t = ((i >> 1) * (k >> 1)) & 15;
+    const float c = kCos[t], s = kCos[(t + 12) & 15];  // sin(x) = cos(x - pi/2)
+    const int cc = ((i & 1) << 1) | (k & 1);
+    return cc == 0 || cc == 3 ? c : (cc == 1 ? s : -s);
+}
+
+__device__ __forceinline__ Dft16Mfma dft16_mfma_setup() {
+    const int lane = int(threadIdx.x & 63u);
+    Dft16Mfma f;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float v = dft16_entry(lane & 31, 8 * kk + 4 * (lane >> 5) + e);
+            const _Float16 hi = _Float16(v);
+            f.ahi[kk][e] = hi;
+            f.alo[kk][e] = _Float16(v - float(hi));
+        }
+    return f;
+}
+
+__device__ __forceinline__ void swap_halves(h4& a, h4& b) {
+    uint32_t ua[2], ub[2];
+    __builtin_memcpy(ua, &a, 8);
+    __builtin_memcpy(ub, &b, 8);
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        const auto r = __builtin_amdgcn_permlane32_swap(ua[w], ub[w], false, false);
+        ua[w] = r[0];
+        ub[w] = r[1];
+    }
+    __builtin_memcpy(&a, ua, 8);
+    __builtin_memcpy(&b, ub, 8);
+}
+
+// Wave-collective: every lane of the wave calls it with its butterfly in v[0..15];
+// returns the DFT in natural order.
+__device__ __forceinline__ void dft16_mfma(cf* v, const Dft16Mfma& f) {
+    float mx = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mx = fmaxf(mx, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    int ex = 0;
+    (void)frexpf(mx, &ex);                          // mx < 2^ex
+    const float scale = mx > 0.0f ? ldexpf(1.0f, 14 - ex) : 1.0f;
+    const float unscale = mx > 0.0f ? ldexpf(1.0f, ex - 14) : 1.0f;
+    f32x16 da = {}, db = {};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        h4 phi, plo, qhi, qlo;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const cf xp = v[4 * kk + (e >> 1)], xq = v[4 * kk + 2 + (e >> 1)];
+            const float a = ((e & 1) ? xp.y : xp.x) * scale, b = ((e & 1) ? xq.y : xq.x) * scale;
+            phi[e] = _Float16(a);
+            plo[e] = _Float16(a - float(phi[e]));
+            qhi[e] = _Float16(b);
+            qlo[e] = _Float16(b - float(qhi[e]));
+        }
+        swap_halves(phi, qhi);  // lanes < 32 now feed butterflies 0..31 through p, 32..63 through q
+        swap_halves(plo, qlo);
+        da = __builtin_amdgcn_mfma_f32_32x32x8f16(f.alo[kk], phi, da, 0, 0, 0);
+        da = __builtin_amdgcn_mfma_f32_32x32x8f16(f.ahi[kk], plo, da, 0, 0, 0);
+        da = __builtin_amdgcn_mfma_f32_32x32x8f16(f.ahi[kk], phi, da, 0, 0, 0);
+        db = __builtin_amdgcn_mfma_f32_32x32x8f16(f.alo[kk], qhi, db, 0, 0, 0);
+        db = __builtin_amdgcn_mfma_f32_32x32x8f16(f.ahi[kk], qlo, db, 0, 0, 0);
+        db = __builtin_amdgcn_mfma_f32_32x32x8f16(f.ahi[kk], qhi, db, 0, 0, 0);
+    }
+    // After the swap every lane holds its own butterfly: rows with even (i / 4) in da,
+    // odd in db, register 4 (i / 8) + i % 4.
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(da[r]), __float_as_uint(db[r]), false, false);
+        da[r] = __uint_as_float(s[0]);
+        db[r] = __uint_as_float(s[1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int i = 2 * k;  // re row i, im row i + 1 (same quad)
+        const int r = 4 * (i >> 3) + (i & 3);
+        const bool odd = (i >> 2) & 1;
+        v[k] = cf{(odd ? db[r] : da[r]) * unscale, (odd ? db[r + 1] : da[r + 1]) * unscale};
+    }
+}
+
 }  // namespace brp_dev

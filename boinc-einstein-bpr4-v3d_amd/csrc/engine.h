@@ -67,6 +67,7 @@ struct FusedArgs {
     uint32_t fp64_phase = 0;
     uint32_t fma = 0;                // flag 2 arithmetic (uniform over a handle)
     uint32_t nonneg = 0;             // every template of the group has omega, psi, dt >= 0
+    uint32_t precision = 0;          // 1: pass-2 radix-16 stages as split-FP16 MFMA GEMMs
     // snapshot (per batch slot, or null)
     float* snap_resampled = nullptr;  // N floats
     int32_t* snap_index = nullptr;    // N int32
@@ -80,6 +81,8 @@ struct FusedArgs {
 // The shape of series length n with pass-1 length c, pass-1 rows per block s1 and pass-2
 // column pairs per block s2 (each 0 = the first match).
 bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShape* shape);
+// Whether the shape's pass-2 plan starts with radix-16 stages (the MFMA variant's stages).
+bool fused_mfma_capable(const FusedShape& s);
 // Every compiled shape of series length n, the default first.
 std::vector<FusedShape> fused_shapes(uint32_t n);
 // One launch each covers all `count` templates of the group.

@@ -213,7 +213,8 @@ void check_options(const brp_params& p, const brp_options& o) {
     require(o.workgroup && o.workgroup <= 256 && o.batch && o.batch <= 4096 &&
                 (!o.tile || (pow2(o.tile) && o.tile >= 64 && o.tile <= 4096 && o.tile <= p.n)),
             "invalid workgroup, FFT tile, or batch");
-    require(o.precision == 0, "split-FP16 precision variant is not available in this build");
+    require(o.precision <= 1, "unknown precision variant");
+    require(!o.precision || o.engine == BRP_ENGINE_FUSED, "the split-FP16 precision variant needs the fused engine");
 }
 
 void check_template(const brp_params& base, const brp_params& next) {
@@ -308,9 +309,25 @@ void add_fused_series(brp_hip& h, const brp_params& p, float* x) {
     brp_hip::Series s;
     s.n = p.n;
     s.unpadded = p.unpadded;
-    if (!brp_engine::fused_supported(p.n, h.opt.split_c, h.opt.split_s1, h.opt.split_s2, &s.shape))
-        require(!h.series.empty() && brp_engine::fused_supported(p.n, 0, 0, 0, &s.shape),
-                "fused engine: unsupported series length (N = 2^13..2^23 or 3*2^15..3*2^22) or split");
+    // The first compiled shape of this length that fits the overrides (and, for the
+    // split-FP16 variant, whose pass-2 plan starts with radix 16); added series fall back to
+    // ignoring the overrides.
+    const auto pick = [&](bool overrides) {
+        for (const auto& cand : brp_engine::fused_shapes(p.n)) {
+            if (overrides && ((h.opt.split_c && cand.c != h.opt.split_c) ||
+                              (h.opt.split_s1 && cand.s1 != h.opt.split_s1) ||
+                              (h.opt.split_s2 && cand.s2 != h.opt.split_s2)))
+                continue;
+            if (h.opt.precision && !brp_engine::fused_mfma_capable(cand)) continue;
+            s.shape = cand;
+            return true;
+        }
+        return false;
+    };
+    if (!pick(true))
+        require(!h.series.empty() && pick(false),
+                h.opt.precision ? "split-FP16 variant: no shape of this length has radix-16 pass-2 stages"
+                                : "fused engine: unsupported series length (N = 2^13..2^23 or 3*2^15..3*2^22) or split");
     const auto& sh = s.shape;
     auto& a = s.tables;
     a.x = x;
@@ -337,6 +354,7 @@ void create_fused(brp_hip& h) {
     a.slot_words = h.slot_words;
     a.params = h.params_dev;
     a.fp64_phase = h.opt.fp64_phase;
+    a.precision = h.opt.precision;
     if (h.lanes == 2) {
         HIPC(hipStreamCreateWithFlags(&h.side, hipStreamNonBlocking));
         HIPC(hipEventCreateWithFlags(&h.fork, hipEventDisableTiming));

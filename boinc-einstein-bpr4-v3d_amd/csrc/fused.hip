@@ -89,8 +89,11 @@ constexpr int lds_size() { return (L + L / 16) * S; }
 // elements each.  On entry v[u R + q] holds input q of butterfly u = (s, j), b = t + T u,
 // s = b mod S, j = b / S, i.e. element j + q L/R.  On exit of the last stage v[u R + q]
 // holds output element j + q L/R of sequence s.
-template <int L, int S, int T, int ST>
-__device__ __forceinline__ void fft_stages(cf* v, float2* lds, const float2* __restrict__ tw, int t) {
+// MF: radix-16 stages as split-FP16 MFMA GEMMs (the precision variant; wave-collective,
+// so every lane of the block runs the same stages).
+template <int L, int S, int T, int ST, bool MF = false>
+__device__ __forceinline__ void fft_stages(cf* v, float2* lds, const float2* __restrict__ tw, int t,
+                                           const Dft16Mfma* mf = nullptr) {
     constexpr Radices pl = radices_for(L);
     constexpr int R = pl.r[ST];
     constexpr int NS = ns_before<L, ST>();
@@ -105,7 +108,8 @@ __device__ __forceinline__ void fft_stages(cf* v, float2* lds, const float2* __r
 #pragma unroll
             for (int q = 1; q < R; ++q) v[u * R + q] = mul(v[u * R + q], load(tw + k * q * (L / (NS * R))));
         }
-        dft<R>(v + u * R);
+        if constexpr (MF && R == 16) dft16_mfma(v + u * R, *mf);
+        else dft<R>(v + u * R);
     }
     if constexpr (ST + 1 < pl.count) {
         __syncthreads();
@@ -127,7 +131,7 @@ __device__ __forceinline__ void fft_stages(cf* v, float2* lds, const float2* __r
 #pragma unroll
             for (int q = 0; q < R2; ++q) v[u * R2 + q] = load(lds + lds_at<S>(j + q * (L / R2), s));
         }
-        fft_stages<L, S, T, ST + 1>(v, lds, tw, t);
+        fft_stages<L, S, T, ST + 1, MF>(v, lds, tw, t, mf);
     }
 }
 
@@ -312,8 +316,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
 
 // ------------------------------------------------------------------ pass 2 --
 // SNAP: also write the spectrum X (validation runs).
-template <int R, int S, int T, bool SNAP>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(FusedArgs a, FusedShape sh) {
+// MF: the split-FP16 MFMA variant of the radix-16 stages (fewer waves per SIMD: the
+// matrix operands and accumulators take registers).
+template <int R, int S, int T, bool SNAP, bool MF>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MF ? 2 : 4))) void k_pass2(FusedArgs a,
+                                                                                            FusedShape sh) {
     constexpr int SS = 2 * S;
     constexpr int E = R * SS / T;
     static_assert(plan_ok<R>(E), "pass-2 radix plan does not fit the block");
@@ -366,7 +373,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     const float mu = float(red[T / 64] / double(p.steps));
     if (w == 0 && t == 0) a.mean[slot] = mu;
 
-    fft_stages<R, SS, T, 0>(v, lds, a.tw_r, t);
+    if constexpr (MF) {
+        const Dft16Mfma mf = dft16_mfma_setup();
+        fft_stages<R, SS, T, 0, true>(v, lds, a.tw_r, t, &mf);
+    } else {
+        fft_stages<R, SS, T, 0>(v, lds, a.tw_r, t);
+    }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < E / RL; ++u) {
@@ -596,8 +608,15 @@ void launch1(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
 }
 template <int R, int S, int T>
 void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
-    if (a.snap_X) k_pass2<R, S, T, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
-    else k_pass2<R, S, T, false><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
+    if constexpr (radices_for(R).r[0] == 16) {
+        if (a.precision) {
+            if (a.snap_X) k_pass2<R, S, T, true, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
+            else k_pass2<R, S, T, false, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
+            return;
+        }
+    }
+    if (a.snap_X) k_pass2<R, S, T, true, false><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
+    else k_pass2<R, S, T, false, false><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
 }
 
 // The compiled kernel instances: f.template operator()<C, S1, T>() for pass 1 and
@@ -667,6 +686,8 @@ constexpr ShapeRow kShapes[] = {
 }  // namespace
 
 namespace brp_engine {
+
+bool fused_mfma_capable(const FusedShape& s) { return radices_for(int(s.r)).r[0] == 16; }
 
 std::vector<FusedShape> fused_shapes(uint32_t n) {
     std::vector<FusedShape> out;

@@ -357,6 +357,38 @@ def test_uneven_workunit_pipeline(brp, oracle, wl):
         assert [r.bin for r in tops[k].final()] == [r.bin for r in ref.final()], n
 
 
+@pytest.mark.parametrize("n", [3 << 20, 1 << 22])
+def test_split_fp16_mfma_variant(brp, oracle, wl, n):
+    """precision = 1: the pass-2 radix-16 stages as split-FP16 MFMA GEMMs.  Indices and
+    the resampled series are unchanged; the power meets the 1e-5 parity bar against the FP64
+    oracle, and its max / RMS error is reported beside the FP32 path's."""
+    w = wl.mixed_radix(count=2, n=n) if n % 3 == 0 else wl.plain_full(seed=9, n=n, threshold_scale=0.3)
+    p = w.templates[0]
+    errs = {}
+    for prec in (0, 1):
+        say(f"[gpu] split-FP16 MFMA variant N={n} precision={prec}")
+        with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True, precision=prec) as g:
+            info = g.device_info()
+            g.set_template(p)
+            g.run()
+            rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"fused/N={n}/precision={prec}")
+        errs[prec] = rep["power"]
+        if prec:
+            assert " R=1024 " in info or " R=2048 " in info  # a shape with radix-16 pass-2 stages
+    say({"fp32": errs[0], "split_fp16_mfma": errs[1]})
+    assert errs[1]["max_rel"] <= 1e-5
+
+
+def test_split_fp16_rejected_without_radix16_stages(brp, wl):
+    """N = 2^21 with C = 2048 leaves R = 512 (radix 8 only): the variant has no stage to run."""
+    w = wl.plain_full(seed=3, n=1 << 21)
+    p = w.templates[0]
+    with pytest.raises(RuntimeError, match="radix-16"):
+        brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, precision=1, split_c=2048)
+    with pytest.raises(RuntimeError, match="fused engine"):
+        brp.GPU(p, w.x, 256, 0, 1, packed=True, precision=1)
+
+
 def test_fp64_phase_variant(brp, oracle, wl):
     """The FP64-phase variant rounds the exact orbit model; its index differences from the
     FP32 LUT rule (the parity mode) are counted and reported, and its spectrum is checked
