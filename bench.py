@@ -79,6 +79,8 @@ def main() -> int:
     ap.add_argument("--plain-templates", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision-templates", type=int, default=256,
+                    help="precision leg: templates at N = 3*2^20, FP32 vs split-FP16 MFMA (0 = skip)")
     ap.add_argument("--uneven-templates", type=int, default=2048,
                     help="uneven workunit leg: templates over N = 2^20, 3*2^19, 2^22 (0 = skip)")
     args = ap.parse_args()
@@ -205,6 +207,27 @@ def main() -> int:
                    "identical": ra == rb and bool(np.allclose(pa, pb, rtol=1e-5, atol=0)) and fa == fb}
         log(f"[bench] top-1000: {top1000}")
 
+    # ---------------------------------------------------------- precision leg --
+    # 3 * 2^20 (radix-3 plans), bench semantics: the FP32 engine and the split-FP16 MFMA
+    # variant over the same templates, one batch each.
+    precision = None
+    if args.precision_templates > 0:
+        wp = workloads.mixed_radix(count=args.precision_templates)
+        precision = {"n": wp.n, "templates": len(wp.templates)}
+        for prec, name in ((0, "fp32"), (1, "split_fp16_mfma")):
+            with brp.GPU(wp.templates[0], wp.x, 256, 0, len(wp.templates), packed=True, engine=brp.ENGINE_FUSED,
+                         graph=not args.no_graph, group=args.group, precision=prec) as gq:
+                gq.set_templates(wp.templates)
+                gq.run()  # warm-up and graph capture
+                tq0 = time.perf_counter()
+                for _ in range(args.steps):
+                    gq.run()
+                tps_q = len(wp.templates) * args.steps / (time.perf_counter() - tq0)
+                shape_q = re.findall(r"C=(\d+) R=(\d+)", gq.device_info())[0]
+            precision[name] = {"value": round(tps_q, 3), "unit": "templates/s", "C": int(shape_q[0]),
+                               "R": int(shape_q[1])}
+        log(f"[bench] precision leg: {precision}")
+
     # ------------------------------------------------------- uneven workunit --
     # 4-bit DD blocks of three lengths (quantized N(7.5, 2^2) noise, a pulsar, 10 RFI
     # bins at 50 sigma) decoded, whitened and RFI-cleaned on the device, then one bank
@@ -293,6 +316,19 @@ def main() -> int:
         power_error = {"max_rel": pe["max_rel"], "rms_rel": pe["rms_rel"],
                        "definition": "|P_gpu - P_fp64| / max(P_fp64, mean P_fp64), bank template 0"}
         log(f"[bench] power error vs FP64 oracle {power_error}")
+        if precision is not None:
+            # The same error for the first template of the precision leg, both paths.
+            p0 = wp.templates[0]
+            p64q = oracle_lib.reference(wp.x, p0)[3]
+            for prec, name in ((0, "fp32"), (1, "split_fp16_mfma")):
+                with brp.GPU(p0, wp.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True,
+                             precision=prec) as gs:
+                    gs.set_template(p0)
+                    gs.run()
+                    pq = parity.power_error(gs.snapshot(0).power, p64q)
+                precision[name]["max_rel"] = pq["max_rel"]
+                precision[name]["rms_rel"] = pq["rms_rel"]
+            log(f"[bench] precision leg errors: {precision}")
 
     out = {
         "metric": "templates/sec at N=2^22, batched vs plain (1 GPU), identical toplist; % roofline",
@@ -320,6 +356,7 @@ def main() -> int:
         "toplist_identical": identical,
         "top1000": top1000,
         "uneven_workunit": uneven,
+        "precision_variant": precision,
         "power_error": power_error,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
