@@ -155,6 +155,23 @@ public:
     }
     brp_hip* handle() { return impl_.get(); }
 
+    // Batched extension (fused engine): another observation on the same handle; templates
+    // name it by (n, unpadded).
+    void add_series(const Params& p, std::span<const float> input) {
+        check(brp_hip_add_series(impl_.get(), &p, input.data(), input.size()));
+    }
+    // src/workunit_main.cpp:77-136 over prepared templates of any registered series, one
+    // toplist per series (registration order); returns the candidate count per template.
+    std::vector<uint32_t> bank_search(std::span<const Params> templates, std::span<const brp_orbit> orbits,
+                                      std::span<brp_toplist* const> tops) {
+        if (templates.size() != orbits.size()) throw std::runtime_error("templates and orbits differ in length");
+        std::vector<uint32_t> counts(templates.size());
+        double gpu_ms = 0;
+        check(brp_hip_bank_search_templates(impl_.get(), templates.data(), orbits.data(), templates.size(),
+                                            tops.data(), tops.size(), counts.data(), &gpu_ms));
+        return counts;
+    }
+
 private:
     struct Free { void operator()(brp_hip* h) const { brp_hip_destroy(h); } };
     std::unique_ptr<brp_hip, Free> impl_;
@@ -162,5 +179,15 @@ private:
     uint32_t batch_;
     std::vector<Params> templates_;
 };
+
+// load_observation's decode + preprocess (src/workload.cpp:29-95) on the device, for the
+// sample block of a DD file; returns the preprocessed series (samples floats).
+inline std::vector<float> preprocess(std::span<const uint8_t> block, size_t samples, uint32_t bits, double scale,
+                                     uint32_t n, uint32_t window = 1000, float rfi_ratio = 30.0f,
+                                     uint32_t* rfi_bins = nullptr) {
+    std::vector<float> out(samples);
+    check(brp_hip_preprocess(block.data(), samples, bits, scale, n, window, rfi_ratio, out.data(), nullptr, rfi_bins));
+    return out;
+}
 
 }  // namespace brp
