@@ -70,3 +70,40 @@ def test_create_without_device_fails_cleanly(brp):
     x = np.zeros(p.unpadded, np.float32)
     with pytest.raises(RuntimeError):
         brp.GPU(p, x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED)
+
+
+def test_cpp_mirror_compiles_links_and_runs_host_science(brp, tmp_path):
+    """include/brp/hip.hpp (the reference-shaped C++ class and science functions) compiles
+    with g++ -std=c++20, links against the library, and its host-only calls agree with the
+    Python binding (no GPU call is made)."""
+    import shutil
+    import subprocess
+
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    src = tmp_path / "mirror.cpp"
+    src.write_text(r'''
+#include "brp/hip.hpp"
+#include <cstdio>
+int main() {
+    brp::Params p = brp::parameters(65536, 65536, 0.05f, 3600.0f, 1.0f, 64e-6f);
+    brp::Toplist top(100);
+    std::printf("%u %u %.9g %.9g\n", p.steps, p.fundamental, double(p.s0), double(brp::lut_sin(1.25f)));
+    try {
+        brp::parameters(100, 100, 0.01f, 100.0f, 0.1f, 64e-6f);
+    } catch (const std::runtime_error& e) {
+        std::printf("error: %s\n", e.what());
+    }
+    return 0;
+}
+''')
+    exe = tmp_path / "mirror"
+    lib_dir = brp.LIBRARY.parent
+    subprocess.run(["g++", "-std=c++20", f"-I{ROOT / 'include'}", str(src), "-o", str(exe), f"-L{lib_dir}",
+                    "-lbrp_hip", f"-Wl,-rpath,{lib_dir}"], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()
+    p = brp.parameters(65536, 65536, 0.05, 3600.0, 1.0, 64e-6)
+    steps, fundamental, s0, lut = out[0].split()
+    assert int(steps) == p.steps and int(fundamental) == p.fundamental
+    assert float(s0) == pytest.approx(p.s0, rel=1e-7) and float(lut) == pytest.approx(brp.lut_sin(1.25), rel=1e-7)
+    assert out[1].startswith("error: invalid observation/template")
