@@ -76,6 +76,8 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner, use --group / --split-c")
     ap.add_argument("--tune-templates", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--timing-stride", type=int, default=0,
+                    help="HIP events on every n-th launch group (0: one group per batch)")
     ap.add_argument("--plain-templates", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -120,9 +122,12 @@ def main() -> int:
         log(f"[bench] autotune picked {best} in {tuned['seconds']} s: {report}")
 
     # ---------------------------------------------------------------- batched --
+    # Kernel events on one launch group per batch: the sampled launches give the per-kernel
+    # durations, and the event nodes cost the timed region little.
+    stride = args.timing_stride or max(1, args.batch // max(args.group, 1))
     g = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
                 graph=not args.no_graph, group=args.group, split_c=args.split_c, split_s1=args.split_s1,
-                split_s2=args.split_s2, lanes=args.lanes)
+                split_s2=args.split_s2, lanes=args.lanes, timing_stride=stride)
     info = g.device_info()
     log("[bench]", info)
     shape = {k: int(v) for k, v in re.findall(r"\b(C|R|S1|S2|group)=(\d+)", info)}
@@ -143,15 +148,16 @@ def main() -> int:
         gpu_ms_total += gpu_ms
         log(f"[bench] step {i + 1}/{args.steps} done")
     elapsed = time.perf_counter() - t_start
-    # Per-kernel event times summed over the timed region, and the launches they cover.
-    km_list, launches = g.kernel_ms()
+    # Per-kernel event times summed over the timed launches of the timed region, the
+    # launches and the templates they cover.
+    km_list, launches, timed_templates = g.kernel_ms()
     km = np.array(km_list)
     templates_per_s = args.templates * args.steps / elapsed
     batched_rows = [(r.bin, r.harmonics, r.period, r.power) for r in top.table()]
     final_bins = [r.bin for r in top.final()]
     g.close()
     log(f"[bench] batched: {templates_per_s:.1f} templates/s; kernel ms over the timed region {km.tolist()} "
-        f"in {launches} launches")
+        f"in {launches} timed launches of {timed_templates} templates (every {stride}th group)")
 
     # The same configuration without per-kernel events in the graphs.
     gu = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
@@ -277,7 +283,7 @@ def main() -> int:
     kms = {"pass1_resample_fft": km[0], "pass2_fft_power": km[1], "harmonic": km[3]}
     dom = max(kms, key=lambda k: kms[k])
     per_launch_ms = kms[dom] / max(launches, 1)
-    templates_per_launch = args.templates * args.steps / max(launches, 1)
+    templates_per_launch = timed_templates / max(launches, 1)
     achieved = kb[dom] * templates_per_launch / (per_launch_ms * 1e-3) if per_launch_ms > 0 else 0.0
     traffic, traffic_src = measured_traffic(dom, n, shape)
     pipeline_achieved = B * templates_per_s
@@ -368,7 +374,8 @@ def main() -> int:
                               "stage_boundary_bytes_per_template": 4 * U + 18 * n + 4 * H,
                               "fft_flop_per_template": round(5 * (n // 2) * np.log2(n // 2)),
                               "fft_tflops": round(5 * (n // 2) * np.log2(n // 2) * templates_per_s / 1e12, 3)},
-        "kernel_us_per_template": {k: round(v * 1e3 / (args.templates * args.steps), 3) for k, v in kms.items()},
+        "kernel_us_per_template": {k: round(v * 1e3 / max(timed_templates, 1), 3) for k, v in kms.items()},
+        "kernel_timing": {"timed_launches": launches, "timed_templates": timed_templates, "every_nth_group": stride},
         "copy_GBps": round(copy_gbps, 1),
         "cpu_baseline": cpu,
     }

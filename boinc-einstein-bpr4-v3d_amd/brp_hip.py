@@ -62,7 +62,7 @@ class Options(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint32) for name in (
         "workgroup", "tile", "batch", "software", "stockham", "radix4", "packed", "scatter", "radix8",
         "fused_power", "radix8_store", "engine", "graph", "snapshot", "fp64_phase", "split_c", "precision",
-        "group", "split_s1", "split_s2", "lanes", "untimed")] + [("reserved", ctypes.c_uint32 * 1)]
+        "group", "split_s1", "split_s2", "lanes", "untimed", "timing_stride")]
 
 
 class Timings(ctypes.Structure):
@@ -120,7 +120,7 @@ def load() -> ctypes.CDLL:
         "brp_hip_device_info": (sz, [ctypes.c_void_p, ctypes.c_char_p, sz]),
         "brp_hip_allocated_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
         "brp_hip_profile_counters": (sz, [ctypes.c_void_p, ctypes.c_char_p, sz]),
-        "brp_hip_kernel_ms": (i32, [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_uint64)]),
+        "brp_hip_kernel_ms": (i32, [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_uint64), P(ctypes.c_uint64)]),
         "brp_hip_copy_bandwidth": (i32, [sz, i32, P(ctypes.c_double)]),
         "brp_hip_add_series": (i32, [ctypes.c_void_p, P(Params), fp, sz]),
         "brp_hip_bank_search_templates": (i32, [ctypes.c_void_p, P(Params), P(Orbit), sz, P(ctypes.c_void_p), sz,
@@ -322,12 +322,12 @@ class GPU:
                  scatter: bool = False, radix8: bool = False, fused_power: bool = False, radix8_store: bool = False,
                  *, engine: int = ENGINE_PLAIN, graph: bool = False, snapshot: bool = False, fp64_phase: bool = False,
                  split_c: int = 0, precision: int = 0, group: int = 0, split_s1: int = 0, split_s2: int = 0,
-                 lanes: int = 0, untimed: bool = False):
+                 lanes: int = 0, untimed: bool = False, timing_stride: int = 0):
         lib = load()
         x = np.ascontiguousarray(input, dtype=np.float32)
         o = Options(workgroup, tile, batch, software, stockham, radix4, packed, scatter, radix8, fused_power,
                     radix8_store, engine, graph, snapshot, fp64_phase, split_c, precision, group, split_s1,
-                    split_s2, lanes, untimed)
+                    split_s2, lanes, untimed, timing_stride)
         self._h = ctypes.c_void_p()
         self.p = p.copy()
         self.batch = batch
@@ -405,13 +405,14 @@ class GPU:
         load().brp_hip_profile_counters(self._h, buf, 1024)
         return buf.value.decode()
 
-    def kernel_ms(self) -> tuple[list[float], int]:
-        """Per-stage device ms since the previous call (which resets them) and the number
-        of launch sequences (groups or templates) they cover."""
+    def kernel_ms(self) -> tuple[list[float], int, int]:
+        """Per-stage device ms since the previous call (which resets them), the number of
+        launch sequences (groups or templates) they cover and the templates in those."""
         out = (ctypes.c_double * 4)()
         n = ctypes.c_uint64()
-        _check(load().brp_hip_kernel_ms(self._h, out, ctypes.byref(n)))
-        return list(out), n.value
+        m = ctypes.c_uint64()
+        _check(load().brp_hip_kernel_ms(self._h, out, ctypes.byref(n), ctypes.byref(m)))
+        return list(out), n.value, m.value
 
     def add_series(self, p: Params, input: np.ndarray) -> None:
         """Registers another observation (fused engine): templates name it by (n, unpadded)."""

@@ -121,6 +121,7 @@ struct brp_hip {
     uint32_t group = 1;
     uint32_t lanes = 1;              // groups in flight, on stream and side
     bool timed = true;               // per-kernel events
+    uint32_t stride = 1;             // on every stride-th launch group
     hipStream_t side = nullptr;      // second lane
     hipEvent_t fork = nullptr, join = nullptr;
     size_t lane_y = 0, lane_p = 0, lane_partial = 0;  // per-lane buffer strides (elements)
@@ -145,7 +146,9 @@ struct brp_hip {
     bool ran[2] = {false, false};
     double submit_ms = 0;
     double total_ms[4] = {0, 0, 0, 0};  // since the last brp_hip_kernel_ms
-    uint64_t total_launches = 0;
+    uint64_t total_launches = 0;        // launch sequences timed since then
+    uint64_t total_templates = 0;       // templates those covered
+    uint32_t timed_launches[2] = {0, 0}, timed_templates[2] = {0, 0};  // of the last run of each half
 
     size_t slot_base(int half) const { return size_t(half) * opt.batch; }
 
@@ -348,6 +351,7 @@ void create_fused(brp_hip& h) {
     h.lanes = h.opt.snapshot ? 1 : (h.opt.lanes ? h.opt.lanes : 1);
     require(h.lanes <= 2, "fused engine: lanes must be 1 or 2");
     h.timed = !h.opt.untimed;
+    h.stride = h.opt.timing_stride ? h.opt.timing_stride : 1;
     auto& a = h.fa;
     a.mean = h.alloc<float>(2 * size_t(h.opt.batch));
     a.results = h.results;
@@ -446,12 +450,14 @@ void enqueue_fused(brp_hip& h, int half, bool capturing) {
         HIPC(hipStreamWaitEvent(h.side, h.fork, 0));
     }
     hipStream_t st = h.stream;
+    uint32_t gi_now = 0;
     auto record = [&](size_t i) {
-        if (!ev) return;
+        if (!ev || gi_now % h.stride) return;
         if (capturing) HIPC(hipEventRecordWithFlags(h.event(half, i), st, hipEventRecordExternal));
         else HIPC(hipEventRecord(h.event(half, i), st));
     };
     for (uint32_t gi = 0; gi < groups.size(); ++gi) {
+        gi_now = gi;
         const auto& grp = groups[gi];
         const auto& ser = h.series[grp.series];
         brp_engine::FusedArgs a = h.fa;
@@ -532,7 +538,9 @@ void fill_timings(brp_hip& h, int half, uint32_t count) {
     per.assign(4 * size_t(count), 0.0);
     std::fill(km, km + 4, 0.0);
     float ms = 0;
+    h.timed_launches[half] = h.timed_templates[half] = 0;
     if (h.opt.engine == BRP_ENGINE_PLAIN) {
+        h.timed_launches[half] = h.timed_templates[half] = count;
         for (uint32_t r = 0; r < count; ++r)
             for (uint32_t s = 0; s < 4; ++s) {
                 HIPC(hipEventElapsedTime(&ms, h.event(half, 5 * r + s), h.event(half, 5 * r + s + 1)));
@@ -543,8 +551,10 @@ void fill_timings(brp_hip& h, int half, uint32_t count) {
     }
     if (!h.timed || (h.opt.graph && !h.opt.snapshot && !h.graph_events)) return;
     const auto& groups = h.groups[half];
-    for (uint32_t gi = 0; gi < groups.size(); ++gi) {
+    for (uint32_t gi = 0; gi < groups.size(); gi += h.stride) {
         const uint32_t first = groups[gi].first, n = groups[gi].count;
+        ++h.timed_launches[half];
+        h.timed_templates[half] += n;
         double stage[4] = {0, 0, 0, 0};
         HIPC(hipEventElapsedTime(&ms, h.event(half, 4 * gi), h.event(half, 4 * gi + 1)));
         stage[0] = ms;
@@ -598,7 +608,8 @@ void finish(brp_hip& h, int half) {
     const uint32_t count = uint32_t(h.templates[half].size());
     fill_timings(h, half, count);
     for (int s = 0; s < 4; ++s) h.total_ms[s] += h.kernel_ms[half][s];
-    h.total_launches += h.opt.engine == BRP_ENGINE_PLAIN ? count : h.groups[half].size();
+    h.total_launches += h.timed_launches[half];
+    h.total_templates += h.timed_templates[half];
 }
 
 void read_slot(brp_hip& h, int half, uint32_t slot, std::vector<brp_candidate>& out) {
@@ -811,12 +822,13 @@ size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap) {
     return text.size();
 }
 
-int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches) {
+int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches, uint64_t* templates) {
     return guarded([&] {
         require(h && out, "null argument");
         for (int s = 0; s < 4; ++s) { out[s] = h->total_ms[s]; h->total_ms[s] = 0; }
         if (launches) *launches = h->total_launches;
-        h->total_launches = 0;
+        if (templates) *templates = h->total_templates;
+        h->total_launches = h->total_templates = 0;
     });
 }
 

@@ -85,7 +85,9 @@ typedef struct brp_options {
     uint32_t lanes;        /* fused engine: groups in flight (1 or 2 streams; 0 = 1), each lane
                               with its own exchange and power buffers */
     uint32_t untimed;      /* fused engine: no per-kernel events (stage times read as 0) */
-    uint32_t reserved[1];
+    uint32_t timing_stride; /* fused engine: per-kernel events on every n-th launch group only
+                               (0 or 1 = every group); stage times and brp_hip_kernel_ms then
+                               cover the timed groups */
 } brp_options;
 
 /* src/vulkan.hpp:6 Timings; template_stage_ms is returned separately. */
@@ -203,9 +205,10 @@ size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap);
 /* Per-stage device times (ms) accumulated over every run since the previous call of this
  * function (which resets them), measured with HIP events on the launch stream:
  * resample/pass 1, fft/pass 2, power, harmonic.  *launches (may be NULL) receives the
- * number of launch sequences timed: groups of templates (fused engine) or templates
- * (plain engine). */
-int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches);
+ * number of launch sequences timed: groups of templates (fused engine; with
+ * options.timing_stride n, every n-th group) or templates (plain engine); *templates (may be
+ * NULL) the templates those launches covered. */
+int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches, uint64_t* templates);
 /* Autotuner (no reference counterpart; the reference picks its radix fusion and tile by
  * hand-run sweeps).  For the series length of templates[0], times the fused engine on the
  * caller's `count` templates (one run of `count` per measurement, `reps` measurements

@@ -228,6 +228,26 @@ def test_fused_two_lanes_match_one(brp, wl, graph):
     assert sum(len(c) for c in ref) > 0
 
 
+def test_sampled_kernel_timing(brp, wl):
+    """timing_stride n: events on every n-th launch group; kernel_ms reports those launches
+    and the templates they covered, and the candidates do not change."""
+    w = wl.bank(count=14, n=1 << 20)
+    say("[gpu] timing stride 1 vs 3, batch 14, group 2, graph")
+    got = []
+    for stride in (1, 3):
+        with brp.GPU(w.templates[0], w.x, 256, 0, 14, packed=True, engine=brp.ENGINE_FUSED, graph=True, group=2,
+                     timing_stride=stride) as g:
+            g.set_templates(w.templates)
+            g.kernel_ms()
+            for _ in range(2):
+                g.run()
+            km, launches, templates = g.kernel_ms()
+            got.append([g.candidates(s) for s in range(14)])
+        assert launches == 2 * len(range(0, 7, stride)) and templates == 2 * 2 * len(range(0, 7, stride))
+        assert all(v > 0 for v in (km[0], km[1], km[3]))
+    assert all(np.array_equal(a, b) for a, b in zip(*got))
+
+
 def _oracle_bank(oracle, x, templates, orbits, rows=100):
     top = oracle.OToplist(rows)
     for p, o in zip(templates, orbits):
