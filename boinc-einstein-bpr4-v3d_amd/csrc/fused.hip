@@ -633,6 +633,8 @@ bool with_pass1(uint32_t c, uint32_t s1, F&& f) {
         case 2048 * 1000 + 4: f.template operator()<2048, 4, 512>(); return true;
         case 4096 * 1000 + 4: f.template operator()<4096, 4, 1024>(); return true;
         case 4096 * 1000 + 2: f.template operator()<4096, 2, 512>(); return true;
+        case 4096 * 1000 + 1: f.template operator()<4096, 1, 256>(); return true;
+        case 2048 * 1000 + 2: f.template operator()<2048, 2, 256>(); return true;
         case 384 * 1000 + 32: f.template operator()<384, 32, 256>(); return true;
         case 768 * 1000 + 16: f.template operator()<768, 16, 256>(); return true;
         case 1536 * 1000 + 8: f.template operator()<1536, 8, 512>(); return true;
@@ -646,8 +648,11 @@ bool with_pass2(uint32_t r, uint32_t s2, F&& f) {
         case 64 * 1000 + 32: f.template operator()<64, 32, 256>(); return true;
         case 128 * 1000 + 16: f.template operator()<128, 16, 256>(); return true;
         case 256 * 1000 + 16: f.template operator()<256, 16, 512>(); return true;
+        case 256 * 1000 + 8: f.template operator()<256, 8, 256>(); return true;
         case 512 * 1000 + 16: f.template operator()<512, 16, 1024>(); return true;
         case 512 * 1000 + 8: f.template operator()<512, 8, 512>(); return true;
+        case 512 * 1000 + 4: f.template operator()<512, 4, 256>(); return true;
+        case 1024 * 1000 + 2: f.template operator()<1024, 2, 256>(); return true;
         case 1024 * 1000 + 8: f.template operator()<1024, 8, 1024>(); return true;
         case 1024 * 1000 + 4: f.template operator()<1024, 4, 512>(); return true;
         case 2048 * 1000 + 4: f.template operator()<2048, 4, 1024>(); return true;
@@ -681,6 +686,9 @@ constexpr ShapeRow kShapes[] = {
     {1u << 19, 2048, 256, 4, 16}, {1u << 20, 4096, 256, 2, 16},
     {1u << 21, 4096, 512, 2, 8}, {1u << 21, 4096, 512, 2, 16}, {1u << 21, 4096, 512, 4, 8},
     {1u << 21, 2048, 1024, 4, 4}, {1u << 22, 4096, 1024, 2, 4},
+    // quarter-size blocks (256 threads, four blocks per CU)
+    {1u << 21, 4096, 512, 1, 4}, {1u << 21, 4096, 512, 2, 4}, {1u << 21, 4096, 512, 1, 8},
+    {1u << 21, 2048, 1024, 2, 2}, {1u << 20, 4096, 256, 1, 8},
 };
 
 }  // namespace

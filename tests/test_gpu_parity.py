@@ -137,7 +137,9 @@ ALTERNATIVE_SPLITS = [
     (1 << 22, 4096, 4, 8), (1 << 22, 2048, 4, 4),
     (1 << 23, 2048, 8, 4), (1 << 23, 4096, 2, 4),
     (3 << 19, 1536, 8, 16), (3 << 19, 768, 16, 8), (3 << 20, 1536, 8, 8), (3 << 20, 768, 16, 4),
-    (3 << 21, 1536, 8, 4)]
+    (3 << 21, 1536, 8, 4),
+    (1 << 22, 4096, 1, 4), (1 << 22, 4096, 2, 4), (1 << 22, 4096, 1, 8), (1 << 22, 2048, 2, 2),
+    (1 << 21, 4096, 1, 8)]
 
 
 @pytest.mark.parametrize("n,c,s1,s2", ALTERNATIVE_SPLITS)
