@@ -62,7 +62,7 @@ class Options(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint32) for name in (
         "workgroup", "tile", "batch", "software", "stockham", "radix4", "packed", "scatter", "radix8",
         "fused_power", "radix8_store", "engine", "graph", "snapshot", "fp64_phase", "split_c", "precision",
-        "group", "split_s1", "split_s2", "lanes", "untimed", "timing_stride")]
+        "group", "split_s1", "split_s2", "lanes", "untimed", "timing_stride", "split_t1", "split_t2")]
 
 
 class Timings(ctypes.Structure):
@@ -234,7 +234,7 @@ def autotune(templates, x: np.ndarray, batch_options: "dict | None" = None, reps
                                    buf, len(buf)))
     report = [json.loads(line) for line in buf.value.decode().splitlines() if line.strip()]
     return {"split_c": best.split_c, "split_s1": best.split_s1, "split_s2": best.split_s2,
-            "group": best.group, "lanes": best.lanes}, report
+            "split_t1": best.split_t1, "split_t2": best.split_t2, "group": best.group, "lanes": best.lanes}, report
 
 
 def copy_bandwidth(nbytes: int, reps: int = 10) -> float:
@@ -322,12 +322,13 @@ class GPU:
                  scatter: bool = False, radix8: bool = False, fused_power: bool = False, radix8_store: bool = False,
                  *, engine: int = ENGINE_PLAIN, graph: bool = False, snapshot: bool = False, fp64_phase: bool = False,
                  split_c: int = 0, precision: int = 0, group: int = 0, split_s1: int = 0, split_s2: int = 0,
-                 lanes: int = 0, untimed: bool = False, timing_stride: int = 0):
+                 lanes: int = 0, untimed: bool = False, timing_stride: int = 0, split_t1: int = 0,
+                 split_t2: int = 0):
         lib = load()
         x = np.ascontiguousarray(input, dtype=np.float32)
         o = Options(workgroup, tile, batch, software, stockham, radix4, packed, scatter, radix8, fused_power,
                     radix8_store, engine, graph, snapshot, fp64_phase, split_c, precision, group, split_s1,
-                    split_s2, lanes, untimed, timing_stride)
+                    split_s2, lanes, untimed, timing_stride, split_t1, split_t2)
         self._h = ctypes.c_void_p()
         self.p = p.copy()
         self.batch = batch

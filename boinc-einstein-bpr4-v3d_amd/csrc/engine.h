@@ -41,6 +41,7 @@ void plain_harmonic(hipStream_t st, const float* power, float* summed, const brp
 struct FusedShape {
     uint32_t n = 0, m = 0, c = 0, r = 0;  // N, M = N/2, pass-1 length C, pass-2 length R
     uint32_t s1 = 0, s2 = 0;              // rows per pass-1 block, lower columns per pass-2 block
+    uint32_t t1 = 0, t2 = 0;              // threads per pass-1 / pass-2 block
     uint32_t blocks1 = 0, blocks2 = 0;    // blocks per template
     uint32_t tile = 1024;                 // harmonic bins per block
 };
@@ -78,9 +79,10 @@ struct FusedArgs {
     uint32_t snap_m1 = 0;             // spectrum stride (largest M + 1)
 };
 
-// The shape of series length n with pass-1 length c, pass-1 rows per block s1 and pass-2
-// column pairs per block s2 (each 0 = the first match).
-bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShape* shape);
+// The shape of series length n with pass-1 length c, pass-1 rows per block s1, pass-2
+// column pairs per block s2 and threads per block t1, t2 (each 0 = the first match).
+bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShape* shape, uint32_t t1 = 0,
+                     uint32_t t2 = 0);
 // Whether the shape's pass-2 plan starts with radix-16 stages (the MFMA variant's stages).
 bool fused_mfma_capable(const FusedShape& s);
 // Every compiled shape of series length n, the default first.

@@ -319,7 +319,9 @@ void add_fused_series(brp_hip& h, const brp_params& p, float* x) {
         for (const auto& cand : brp_engine::fused_shapes(p.n)) {
             if (overrides && ((h.opt.split_c && cand.c != h.opt.split_c) ||
                               (h.opt.split_s1 && cand.s1 != h.opt.split_s1) ||
-                              (h.opt.split_s2 && cand.s2 != h.opt.split_s2)))
+                              (h.opt.split_s2 && cand.s2 != h.opt.split_s2) ||
+                              (h.opt.split_t1 && cand.t1 != h.opt.split_t1) ||
+                              (h.opt.split_t2 && cand.t2 != h.opt.split_t2)))
                 continue;
             if (h.opt.precision && !brp_engine::fused_mfma_capable(cand)) continue;
             s.shape = cand;
@@ -794,7 +796,7 @@ size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap) {
         s << "; engine=fused";
         for (const auto& ser : h->series)
             s << " N=" << ser.n << " C=" << ser.shape.c << " R=" << ser.shape.r << " S1=" << ser.shape.s1
-              << " S2=" << ser.shape.s2 << ";";
+              << " S2=" << ser.shape.s2 << " T1=" << ser.shape.t1 << " T2=" << ser.shape.t2 << ";";
         s << " group=" << h->group << " lanes=" << h->lanes;
     }
     else if (h)
@@ -856,6 +858,8 @@ int brp_hip_autotune(const brp_params* templates, size_t count, const float* inp
                     o.split_c = s.c;
                     o.split_s1 = s.s1;
                     o.split_s2 = s.s2;
+                    o.split_t1 = s.t1;
+                    o.split_t2 = s.t2;
                     o.group = group;
                     o.lanes = lanes;
                     brp_hip* h = nullptr;
@@ -876,12 +880,15 @@ int brp_hip_autotune(const brp_params* templates, size_t count, const float* inp
                     std::sort(us.begin(), us.end());
                     const double med = us[us.size() / 2];
                     log << "{\"C\":" << s.c << ",\"R\":" << s.r << ",\"S1\":" << s.s1 << ",\"S2\":" << s.s2
+                        << ",\"T1\":" << s.t1 << ",\"T2\":" << s.t2
                         << ",\"group\":" << group << ",\"lanes\":" << lanes << ",\"us_per_template\":" << med << "}\n";
                     if (med < best_us) {
                         best_us = med;
                         best->split_c = s.c;
                         best->split_s1 = s.s1;
                         best->split_s2 = s.s2;
+                        best->split_t1 = s.t1;
+                        best->split_t2 = s.t2;
                         best->group = group;
                         best->lanes = lanes;
                     }

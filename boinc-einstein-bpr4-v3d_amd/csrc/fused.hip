@@ -53,16 +53,40 @@ constexpr Radices radices_for(int L) {
     }
 }
 
-template <int L, int ST>
+// The radix plan of an L-point transform whose threads hold E elements: the table plan
+// when every radix divides E, else the factor 3 first and then radix 16, 8, 4 or 2 as
+// E allows (for example 8 8 8 8 for L = 4096, E = 8).
+constexpr Radices plan_for(int L, int E) {
+    const Radices base = radices_for(L);
+    bool ok = base.count > 0;
+    int prod = 1;
+    for (int i = 0; i < base.count; ++i) {
+        if (E % base.r[i]) ok = false;
+        prod *= base.r[i];
+    }
+    if (ok && prod == L) return base;
+    Radices p{0, {0, 0, 0, 0, 0}};
+    int left = L;
+    if (left % 3 == 0 && E % 3 == 0) { p.r[p.count++] = 3; left /= 3; }
+    while (left > 1 && p.count < 5) {
+        const int r = (left % 16 == 0 && E % 16 == 0) ? 16
+                      : (left % 8 == 0 && E % 8 == 0) ? 8
+                      : (left % 4 == 0 && E % 4 == 0) ? 4 : 2;
+        p.r[p.count++] = r;
+        left /= r;
+    }
+    return left == 1 ? p : Radices{0, {0, 0, 0, 0, 0}};
+}
+
+template <int L, int E, int ST>
 constexpr int ns_before() {
     int p = 1;
-    for (int i = 0; i < ST; ++i) p *= radices_for(L).r[i];
+    for (int i = 0; i < ST; ++i) p *= plan_for(L, E).r[i];
     return p;
 }
 
-template <int L>
-constexpr bool plan_ok(int e) {
-    constexpr Radices pl = radices_for(L);
+constexpr bool plan_ok(int L, int e) {
+    const Radices pl = plan_for(L, e);
     int prod = 1;
     for (int i = 0; i < pl.count; ++i) {
         if (e % pl.r[i]) return false;
@@ -94,10 +118,10 @@ constexpr int lds_size() { return (L + L / 16) * S; }
 template <int L, int S, int T, int ST, bool MF = false>
 __device__ __forceinline__ void fft_stages(cf* v, float2* lds, const float2* __restrict__ tw, int t,
                                            const Dft16Mfma* mf = nullptr) {
-    constexpr Radices pl = radices_for(L);
-    constexpr int R = pl.r[ST];
-    constexpr int NS = ns_before<L, ST>();
     constexpr int E = L * S / T;
+    constexpr Radices pl = plan_for(L, E);
+    constexpr int R = pl.r[ST];
+    constexpr int NS = ns_before<L, E, ST>();
     constexpr int U = E / R;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -156,7 +180,7 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
                                                const float4* __restrict__ lut4, uint32_t r0, uint32_t slot,
                                                int t) {
     constexpr int E = C * S / T;
-    constexpr int R0 = radices_for(C).r[0];
+    constexpr int R0 = plan_for(C, E).r[0];
     const uint32_t R = sh.r;
     const uint32_t steps = p.steps, U = p.unpadded;
     const Orbit32 o{p.tau, p.omega, p.psi, p.dt, p.step_inv, p.s0, p.flags};
@@ -253,12 +277,14 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
 // SNAP: also write the resampled samples and their source indices (validation runs).
 // F64: orbital offset from the FP64 sine (a reported variant; the FP32 LUT rule is the
 // parity mode).
-// At least 4 waves per SIMD: blocks of 512 threads then run two per CU (128 VGPRs).
+// At least 4 waves per SIMD (128 VGPRs): blocks of 512 threads then run two per CU; with
+// eight elements per thread, 6 waves per SIMD (80 VGPRs), three 512-thread blocks per CU.
 template <int C, int S, int T, bool SNAP, bool F64, bool NN, bool FM>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_pass1(FusedArgs a, FusedShape sh) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 8 && !F64 ? 6 : 4))) void k_pass1(
+    FusedArgs a, FusedShape sh) {
     constexpr int E = C * S / T;
-    static_assert(plan_ok<C>(E), "pass-1 radix plan does not fit the block");
-    constexpr Radices pl = radices_for(C);
+    static_assert(plan_ok(C, E), "pass-1 radix plan does not fit the block");
+    constexpr Radices pl = plan_for(C, E);
     constexpr int RL = pl.r[pl.count - 1];
     __shared__ float2 lds[lds_size<C, S>()];
     __shared__ float4 lut4[65];
@@ -319,12 +345,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_p
 // MF: the split-FP16 MFMA variant of the radix-16 stages (fewer waves per SIMD: the
 // matrix operands and accumulators take registers).
 template <int R, int S, int T, bool SNAP, bool MF>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MF ? 2 : 4))) void k_pass2(FusedArgs a,
-                                                                                            FusedShape sh) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MF ? 2 : (2 * R * S / T <= 8 ? 8 : 4)))) void
+k_pass2(FusedArgs a, FusedShape sh) {
     constexpr int SS = 2 * S;
     constexpr int E = R * SS / T;
-    static_assert(plan_ok<R>(E), "pass-2 radix plan does not fit the block");
-    constexpr Radices pl = radices_for(R);
+    static_assert(plan_ok(R, E), "pass-2 radix plan does not fit the block");
+    constexpr Radices pl = plan_for(R, E);
     constexpr int R0 = pl.r[0], RL = pl.r[pl.count - 1];
     __shared__ float2 lds[lds_size<R, SS>()];
     __shared__ double red[T / 64 + 1];
@@ -608,7 +634,7 @@ void launch1(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
 }
 template <int R, int S, int T>
 void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
-    if constexpr (radices_for(R).r[0] == 16) {
+    if constexpr (plan_for(R, R * 2 * S / T).r[0] == 16) {
         if (a.precision) {
             if (a.snap_X) k_pass2<R, S, T, true, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
             else k_pass2<R, S, T, false, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
@@ -620,115 +646,123 @@ void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
 }
 
 // The compiled kernel instances: f.template operator()<C, S1, T>() for pass 1 and
-// <R, S2, T> for pass 2; false when (length, rows) has no instance.
+// <R, S2, T> for pass 2; false when (length, rows, threads) has no instance.
+constexpr uint64_t inst(uint32_t len, uint32_t rows, uint32_t threads) {
+    return (uint64_t(len) << 32) | (uint64_t(rows) << 16) | threads;
+}
 template <typename F>
-bool with_pass1(uint32_t c, uint32_t s1, F&& f) {
-    switch (c * 1000 + s1) {
-        case 64 * 1000 + 64: f.template operator()<64, 64, 256>(); return true;
-        case 128 * 1000 + 32: f.template operator()<128, 32, 256>(); return true;
-        case 256 * 1000 + 16: f.template operator()<256, 16, 256>(); return true;
-        case 512 * 1000 + 16: f.template operator()<512, 16, 512>(); return true;
-        case 1024 * 1000 + 16: f.template operator()<1024, 16, 1024>(); return true;
-        case 2048 * 1000 + 8: f.template operator()<2048, 8, 1024>(); return true;
-        case 2048 * 1000 + 4: f.template operator()<2048, 4, 512>(); return true;
-        case 4096 * 1000 + 4: f.template operator()<4096, 4, 1024>(); return true;
-        case 4096 * 1000 + 2: f.template operator()<4096, 2, 512>(); return true;
-        case 4096 * 1000 + 1: f.template operator()<4096, 1, 256>(); return true;
-        case 2048 * 1000 + 2: f.template operator()<2048, 2, 256>(); return true;
-        case 384 * 1000 + 32: f.template operator()<384, 32, 256>(); return true;
-        case 768 * 1000 + 16: f.template operator()<768, 16, 256>(); return true;
-        case 1536 * 1000 + 8: f.template operator()<1536, 8, 512>(); return true;
-        case 3072 * 1000 + 4: f.template operator()<3072, 4, 256>(); return true;
+bool with_pass1(uint32_t c, uint32_t s1, uint32_t t1, F&& f) {
+    switch (inst(c, s1, t1)) {
+#define BRP_P1(C, S, T) case inst(C, S, T): f.template operator()<C, S, T>(); return true;
+        BRP_P1(64, 64, 256) BRP_P1(128, 32, 256) BRP_P1(256, 16, 256) BRP_P1(512, 16, 512)
+        BRP_P1(1024, 16, 1024) BRP_P1(2048, 8, 1024) BRP_P1(2048, 4, 512) BRP_P1(2048, 2, 256)
+        BRP_P1(4096, 4, 1024) BRP_P1(4096, 2, 512) BRP_P1(4096, 1, 256)
+        BRP_P1(4096, 2, 1024) BRP_P1(4096, 1, 512)
+        BRP_P1(384, 32, 256) BRP_P1(768, 16, 256) BRP_P1(1536, 8, 512) BRP_P1(3072, 4, 256)
+#undef BRP_P1
         default: return false;
     }
 }
 template <typename F>
-bool with_pass2(uint32_t r, uint32_t s2, F&& f) {
-    switch (r * 1000 + s2) {
-        case 64 * 1000 + 32: f.template operator()<64, 32, 256>(); return true;
-        case 128 * 1000 + 16: f.template operator()<128, 16, 256>(); return true;
-        case 256 * 1000 + 16: f.template operator()<256, 16, 512>(); return true;
-        case 256 * 1000 + 8: f.template operator()<256, 8, 256>(); return true;
-        case 512 * 1000 + 16: f.template operator()<512, 16, 1024>(); return true;
-        case 512 * 1000 + 8: f.template operator()<512, 8, 512>(); return true;
-        case 512 * 1000 + 4: f.template operator()<512, 4, 256>(); return true;
-        case 1024 * 1000 + 2: f.template operator()<1024, 2, 256>(); return true;
-        case 1024 * 1000 + 8: f.template operator()<1024, 8, 1024>(); return true;
-        case 1024 * 1000 + 4: f.template operator()<1024, 4, 512>(); return true;
-        case 2048 * 1000 + 4: f.template operator()<2048, 4, 1024>(); return true;
+bool with_pass2(uint32_t r, uint32_t s2, uint32_t t2, F&& f) {
+    switch (inst(r, s2, t2)) {
+#define BRP_P2(R, S, T) case inst(R, S, T): f.template operator()<R, S, T>(); return true;
+        BRP_P2(64, 32, 256) BRP_P2(128, 16, 256) BRP_P2(256, 16, 512) BRP_P2(256, 8, 256)
+        BRP_P2(512, 16, 1024) BRP_P2(512, 8, 512) BRP_P2(512, 4, 256) BRP_P2(512, 4, 512)
+        BRP_P2(1024, 8, 1024) BRP_P2(1024, 4, 512) BRP_P2(1024, 2, 256) BRP_P2(2048, 4, 1024)
+#undef BRP_P2
         default: return false;
     }
 }
 
 // Per series length M = N/2: pass-1 length C with S1 rows per block, pass-2 length R
-// with S2 column pairs per block.  Blocks hold 16 elements per thread where the radix
+// with S2 column pairs per block, and the threads of each block.  Blocks hold 16 elements per thread where the radix
 // plan allows (up to 1024 threads, 16 waves per CU), 48 for the 3 * 2^k plans.  The first
 // row of a length is the default; the others are the autotuner's alternatives (other
 // LDS tiles C x S1 and 2 R x S2, and so other radix sequences).
-struct ShapeRow { uint32_t m, c, r, s1, s2; };
+struct ShapeRow { uint32_t m, c, r, s1, s2, t1, t2; };
 constexpr ShapeRow kShapes[] = {
-    {1u << 12, 64, 64, 64, 32},   {1u << 13, 128, 64, 32, 32},  {1u << 14, 128, 128, 32, 16},
-    {1u << 15, 256, 128, 16, 16}, {1u << 16, 256, 256, 16, 16}, {1u << 17, 512, 256, 16, 16},
-    {1u << 18, 1024, 256, 16, 16}, {1u << 19, 2048, 256, 8, 16}, {1u << 20, 4096, 256, 4, 16},
-    {1u << 21, 4096, 512, 4, 16}, {1u << 22, 4096, 1024, 4, 8},
-    {3u << 14, 384, 128, 32, 16}, {3u << 15, 768, 128, 16, 16}, {3u << 16, 768, 256, 16, 16},
-    {3u << 17, 1536, 256, 8, 16}, {3u << 18, 3072, 256, 4, 16}, {3u << 19, 3072, 512, 4, 16},
-    {3u << 20, 3072, 1024, 4, 8}, {3u << 21, 3072, 2048, 4, 4},
+    {1u << 12, 64, 64, 64, 32, 256, 256},     {1u << 13, 128, 64, 32, 32, 256, 256},
+    {1u << 14, 128, 128, 32, 16, 256, 256},   {1u << 15, 256, 128, 16, 16, 256, 256},
+    {1u << 16, 256, 256, 16, 16, 256, 512},   {1u << 17, 512, 256, 16, 16, 512, 512},
+    {1u << 18, 1024, 256, 16, 16, 1024, 512}, {1u << 19, 2048, 256, 8, 16, 1024, 512},
+    {1u << 20, 4096, 256, 4, 16, 1024, 512},  {1u << 21, 4096, 512, 2, 4, 512, 256},
+    {1u << 22, 4096, 1024, 4, 8, 1024, 1024},
+    {3u << 14, 384, 128, 32, 16, 256, 256},   {3u << 15, 768, 128, 16, 16, 256, 256},
+    {3u << 16, 768, 256, 16, 16, 256, 512},   {3u << 17, 1536, 256, 8, 16, 512, 512},
+    {3u << 18, 3072, 256, 4, 16, 256, 512},   {3u << 19, 3072, 512, 4, 16, 256, 1024},
+    {3u << 20, 3072, 1024, 4, 8, 256, 1024},  {3u << 21, 3072, 2048, 4, 4, 256, 1024},
     // alternatives
-    {1u << 19, 1024, 512, 16, 16}, {1u << 19, 4096, 128, 4, 16},
-    {1u << 20, 2048, 512, 8, 16}, {1u << 20, 1024, 1024, 16, 8},
-    {1u << 21, 2048, 1024, 8, 8}, {1u << 21, 1024, 2048, 16, 4},
-    {1u << 22, 2048, 2048, 8, 4},
-    {3u << 18, 1536, 512, 8, 16}, {3u << 18, 768, 1024, 16, 8},
-    {3u << 19, 1536, 1024, 8, 8}, {3u << 19, 768, 2048, 16, 4},
-    {3u << 20, 1536, 2048, 8, 4},
+    {1u << 21, 4096, 512, 4, 16, 1024, 1024},
+    {1u << 19, 1024, 512, 16, 16, 1024, 1024}, {1u << 19, 4096, 128, 4, 16, 1024, 256},
+    {1u << 20, 2048, 512, 8, 16, 1024, 1024},  {1u << 20, 1024, 1024, 16, 8, 1024, 1024},
+    {1u << 21, 2048, 1024, 8, 8, 1024, 1024},  {1u << 21, 1024, 2048, 16, 4, 1024, 1024},
+    {1u << 22, 2048, 2048, 8, 4, 1024, 1024},
+    {3u << 18, 1536, 512, 8, 16, 512, 1024},   {3u << 18, 768, 1024, 16, 8, 256, 1024},
+    {3u << 19, 1536, 1024, 8, 8, 512, 1024},   {3u << 19, 768, 2048, 16, 4, 256, 1024},
+    {3u << 20, 1536, 2048, 8, 4, 512, 1024},
     // half-size blocks (512 threads, two blocks per CU)
-    {1u << 19, 2048, 256, 4, 16}, {1u << 20, 4096, 256, 2, 16},
-    {1u << 21, 4096, 512, 2, 8}, {1u << 21, 4096, 512, 2, 16}, {1u << 21, 4096, 512, 4, 8},
-    {1u << 21, 2048, 1024, 4, 4}, {1u << 22, 4096, 1024, 2, 4},
+    {1u << 19, 2048, 256, 4, 16, 512, 512},    {1u << 20, 4096, 256, 2, 16, 512, 512},
+    {1u << 21, 4096, 512, 2, 8, 512, 512},     {1u << 21, 4096, 512, 2, 16, 512, 1024},
+    {1u << 21, 4096, 512, 4, 8, 1024, 512},    {1u << 21, 2048, 1024, 4, 4, 512, 512},
+    {1u << 22, 4096, 1024, 2, 4, 512, 512},
     // quarter-size blocks (256 threads, four blocks per CU)
-    {1u << 21, 4096, 512, 1, 4}, {1u << 21, 4096, 512, 2, 4}, {1u << 21, 4096, 512, 1, 8},
-    {1u << 21, 2048, 1024, 2, 2}, {1u << 20, 4096, 256, 1, 8},
+    {1u << 21, 4096, 512, 1, 4, 256, 256},
+    {1u << 21, 4096, 512, 1, 8, 256, 512},     {1u << 21, 2048, 1024, 2, 2, 256, 256},
+    {1u << 20, 4096, 256, 1, 8, 256, 256},
+    // eight elements per thread (radix-8 plans, fewer registers, more waves per CU)
+    {1u << 21, 4096, 512, 2, 4, 1024, 256},    {1u << 21, 4096, 512, 1, 4, 512, 256},
+    {1u << 21, 4096, 512, 2, 4, 512, 512},     {1u << 21, 4096, 512, 1, 4, 512, 512},
 };
 
 }  // namespace
 
 namespace brp_engine {
 
-bool fused_mfma_capable(const FusedShape& s) { return radices_for(int(s.r)).r[0] == 16; }
+bool fused_mfma_capable(const FusedShape& s) { return plan_for(int(s.r), int(2 * s.r * s.s2 / s.t2)).r[0] == 16; }
+
+namespace {
+// The shape of a table row, if both of its kernels are compiled.
+bool shape_of(const ShapeRow& row, uint32_t n, FusedShape* out) {
+    if (!with_pass1(row.c, row.s1, row.t1, []<int, int, int>() {}) ||
+        !with_pass2(row.r, row.s2, row.t2, []<int, int, int>() {}))
+        return false;
+    FusedShape s;
+    s.n = n; s.m = n / 2; s.c = row.c; s.r = row.r; s.s1 = row.s1; s.s2 = row.s2; s.t1 = row.t1; s.t2 = row.t2;
+    s.blocks1 = row.r / row.s1;
+    s.blocks2 = row.c / (2 * row.s2) + 1;
+    *out = s;
+    return true;
+}
+}  // namespace
 
 std::vector<FusedShape> fused_shapes(uint32_t n) {
     std::vector<FusedShape> out;
     for (const ShapeRow& row : kShapes) {
         FusedShape s;
-        if (row.m == n / 2 && fused_supported(n, row.c, row.s1, row.s2, &s)) out.push_back(s);
+        if (row.m == n / 2 && shape_of(row, n, &s)) out.push_back(s);
     }
     return out;
 }
 
-bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShape* out) {
+bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShape* out, uint32_t t1, uint32_t t2) {
     const uint32_t m = n / 2;
     for (const ShapeRow& row : kShapes) {
         if (row.m != m) continue;
-        if ((c && c != row.c) || (s1 && s1 != row.s1) || (s2 && s2 != row.s2)) continue;
-        if (!with_pass1(row.c, row.s1, []<int, int, int>() {}) || !with_pass2(row.r, row.s2, []<int, int, int>() {}))
+        if ((c && c != row.c) || (s1 && s1 != row.s1) || (s2 && s2 != row.s2) || (t1 && t1 != row.t1) ||
+            (t2 && t2 != row.t2))
             continue;
-        FusedShape s;
-        s.n = n; s.m = m; s.c = row.c; s.r = row.r; s.s1 = row.s1; s.s2 = row.s2;
-        s.blocks1 = row.r / row.s1;
-        s.blocks2 = row.c / (2 * row.s2) + 1;
-        *out = s;
-        return true;
+        if (shape_of(row, n, out)) return true;
     }
     return false;
 }
 
 void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
-    with_pass1(s.c, s.s1, [&]<int C, int S, int T>() { launch1<C, S, T>(st, s, a); });
+    with_pass1(s.c, s.s1, s.t1, [&]<int C, int S, int T>() { launch1<C, S, T>(st, s, a); });
 }
 
 void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
-    with_pass2(s.r, s.s2, [&]<int R, int S, int T>() { launch2<R, S, T>(st, s, a); });
+    with_pass2(s.r, s.s2, s.t2, [&]<int R, int S, int T>() { launch2<R, S, T>(st, s, a); });
 }
 
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles) {

@@ -88,6 +88,8 @@ typedef struct brp_options {
     uint32_t timing_stride; /* fused engine: per-kernel events on every n-th launch group only
                                (0 or 1 = every group); stage times and brp_hip_kernel_ms then
                                cover the timed groups */
+    uint32_t split_t1;     /* fused engine: threads per pass-1 block (0 = the first shape that fits) */
+    uint32_t split_t2;     /* fused engine: threads per pass-2 block (0 = the first shape that fits) */
 } brp_options;
 
 /* src/vulkan.hpp:6 Timings; template_stage_ms is returned separately. */
@@ -216,8 +218,8 @@ int brp_hip_kernel_ms(brp_hip* h, double out[4], uint64_t* launches, uint64_t* t
  * rows per block S1, pass-2 length R and column pairs per block S2: the LDS tiles, block
  * sizes and radix sequences), every templates-per-launch value in {2, 4, 8, 16} not above
  * count, and 1 or 2 lanes (only base->lanes when that is nonzero).
- * *best receives `base` with split_c, split_s1, split_s2, group and lanes (1, 2) set to the
- * fastest; report (may be NULL)
+ * *best receives `base` with split_c, split_s1, split_s2, split_t1, split_t2, group and lanes
+ * (1, 2) set to the fastest; report (may be NULL)
  * receives one JSON object per candidate, newline separated. */
 int brp_hip_autotune(const brp_params* templates, size_t count, const float* input, size_t input_count,
                      const brp_options* base, uint32_t reps, brp_options* best, char* report, size_t cap);

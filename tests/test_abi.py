@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(brp):
 def test_struct_layouts(brp):
     assert ctypes.sizeof(brp.Params) == 108
     assert ctypes.sizeof(brp.Candidate) == 16
-    assert ctypes.sizeof(brp.Options) == 23 * 4
+    assert ctypes.sizeof(brp.Options) == 25 * 4
     assert ctypes.sizeof(brp.BankRow) == 48
 
 

@@ -139,21 +139,28 @@ ALTERNATIVE_SPLITS = [
     (3 << 19, 1536, 8, 16), (3 << 19, 768, 16, 8), (3 << 20, 1536, 8, 8), (3 << 20, 768, 16, 4),
     (3 << 21, 1536, 8, 4),
     (1 << 22, 4096, 1, 4), (1 << 22, 4096, 2, 4), (1 << 22, 4096, 1, 8), (1 << 22, 2048, 2, 2),
-    (1 << 21, 4096, 1, 8)]
+    (1 << 21, 4096, 1, 8), (1 << 22, 4096, 4, 16),
+    # eight elements per thread (radix-8 plans): (N, C, S1, S2, T1, T2)
+    (1 << 22, 4096, 2, 4, 1024, 256), (1 << 22, 4096, 1, 4, 512, 256), (1 << 22, 4096, 2, 4, 512, 512),
+    (1 << 22, 4096, 1, 4, 512, 512)]
 
 
-@pytest.mark.parametrize("n,c,s1,s2", ALTERNATIVE_SPLITS)
-def test_fused_alternative_splits(brp, oracle, wl, n, c, s1, s2):
+@pytest.mark.parametrize("shape", ALTERNATIVE_SPLITS)
+def test_fused_alternative_splits(brp, oracle, wl, shape):
     """Every non-default four-step shape the autotuner may pick meets the same bar."""
+    n, c, s1, s2 = shape[:4]
+    t1, t2 = shape[4:] if len(shape) > 4 else (0, 0)
     w = wl.plain_full(seed=7, n=n, threshold_scale=0.3)
     p = w.templates[0]
-    say(f"[gpu] fused N={n} split C={c} S1={s1} S2={s2}")
+    say(f"[gpu] fused N={n} split C={c} S1={s1} S2={s2} T1={t1} T2={t2}")
     with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True, split_c=c,
-                 split_s1=s1, split_s2=s2) as g:
+                 split_s1=s1, split_s2=s2, split_t1=t1, split_t2=t2) as g:
         assert f"C={c} " in g.device_info() and f"S1={s1} S2={s2}" in g.device_info()
+        if t1:
+            assert f"T1={t1} T2={t2}" in g.device_info()
         g.set_template(p)
         g.run()
-        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"fused/N={n}/C={c}/S1={s1}/S2={s2}")
+        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"fused/N={n}/C={c}/S1={s1}/S2={s2}/T={t1},{t2}")
     say(rep)
 
 
@@ -164,12 +171,13 @@ def test_autotune_picks_a_correct_configuration(brp, wl):
     say("[gpu] autotune N=2^20, 8 templates")
     best, report = brp.autotune(w.templates, w.x, reps=2)
     say(best, report)
-    shapes = {(r["C"], r["S1"], r["S2"]) for r in report}
+    shapes = {(r["C"], r["S1"], r["S2"], r["T1"], r["T2"]) for r in report}
     # every shape x (group, lanes) in (2, 1), (2, 2), (4, 1), (4, 2), (8, 1)
     assert len(report) == 5 * len(shapes) and len(shapes) >= 3
     assert {(r["group"], r["lanes"]) for r in report} == {(2, 1), (2, 2), (4, 1), (4, 2), (8, 1)}
-    assert {c for c, _, _ in shapes} == {4096, 2048, 1024}
-    assert (best["split_c"], best["split_s1"], best["split_s2"]) in shapes and best["group"] in {2, 4, 8}
+    assert {s[0] for s in shapes} == {4096, 2048, 1024}
+    assert (best["split_c"], best["split_s1"], best["split_s2"], best["split_t1"], best["split_t2"]) in shapes
+    assert best["group"] in {2, 4, 8}
     assert best["lanes"] in {1, 2}
     assert all(r["us_per_template"] > 0 for r in report)
     out = []
