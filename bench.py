@@ -230,24 +230,35 @@ def main() -> int:
         log(f"[bench] top-1000: {top1000}")
 
     # ---------------------------------------------------------- precision leg --
-    # 3 * 2^20 (radix-3 plans), bench semantics: the FP32 engine and the split-FP16 MFMA
-    # variant over the same templates, one batch each.
+    # 3 * 2^20 (radix-3 plans), bench semantics, one batch each over the same templates:
+    # the split-FP16 MFMA variant (its shape needs a radix-16 pass-2 plan), the FP32 engine
+    # on that same shape, and the FP32 engine on its default shape.
     precision = None
+    precision_kw = {}
     if args.precision_templates > 0:
         wp = workloads.mixed_radix(count=args.precision_templates)
         precision = {"n": wp.n, "templates": len(wp.templates)}
-        for prec, name in ((0, "fp32"), (1, "split_fp16_mfma")):
+        precision_kw["split_fp16_mfma"] = dict(precision=1)
+        for name in ("split_fp16_mfma", "fp32_same_shape", "fp32"):
+            kw = precision_kw.get(name, {})
             with brp.GPU(wp.templates[0], wp.x, 256, 0, len(wp.templates), packed=True, engine=brp.ENGINE_FUSED,
-                         graph=not args.no_graph, group=args.group, precision=prec) as gq:
+                         graph=not args.no_graph, group=args.group, **kw) as gq:
                 gq.set_templates(wp.templates)
                 gq.run()  # warm-up and graph capture
                 tq0 = time.perf_counter()
                 for _ in range(args.steps):
                     gq.run()
                 tps_q = len(wp.templates) * args.steps / (time.perf_counter() - tq0)
-                shape_q = re.findall(r"C=(\d+) R=(\d+)", gq.device_info())[0]
-            precision[name] = {"value": round(tps_q, 3), "unit": "templates/s", "C": int(shape_q[0]),
-                               "R": int(shape_q[1])}
+                info = gq.device_info()
+            shape_q = [int(v) for v in re.findall(r"C=(\d+) R=(\d+)", info)[0]]
+            split_q = [int(v) for v in re.findall(r"S1=(\d+) S2=(\d+)", info)[0]]
+            threads_q = [int(v) for v in re.findall(r"T1=(\d+) T2=(\d+)", info)[0]]
+            if name == "split_fp16_mfma":
+                precision_kw["fp32_same_shape"] = dict(precision=0, split_c=shape_q[0], split_s1=split_q[0],
+                                                       split_s2=split_q[1], split_t1=threads_q[0],
+                                                       split_t2=threads_q[1])
+            precision[name] = {"value": round(tps_q, 3), "unit": "templates/s", "C": shape_q[0], "R": shape_q[1],
+                               "S1": split_q[0], "S2": split_q[1], "T1": threads_q[0], "T2": threads_q[1]}
         log(f"[bench] precision leg: {precision}")
 
     # ------------------------------------------------------- uneven workunit --
@@ -342,9 +353,9 @@ def main() -> int:
             # The same error for the first template of the precision leg, both paths.
             p0 = wp.templates[0]
             p64q = oracle_lib.reference(wp.x, p0)[3]
-            for prec, name in ((0, "fp32"), (1, "split_fp16_mfma")):
+            for name in ("split_fp16_mfma", "fp32_same_shape", "fp32"):
                 with brp.GPU(p0, wp.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True,
-                             precision=prec) as gs:
+                             **precision_kw.get(name, {})) as gs:
                     gs.set_template(p0)
                     gs.run()
                     pq = parity.power_error(gs.snapshot(0).power, p64q)

@@ -658,7 +658,7 @@ bool with_pass1(uint32_t c, uint32_t s1, uint32_t t1, F&& f) {
         BRP_P1(1024, 16, 1024) BRP_P1(2048, 8, 1024) BRP_P1(2048, 4, 512) BRP_P1(2048, 2, 256)
         BRP_P1(4096, 4, 1024) BRP_P1(4096, 2, 512) BRP_P1(4096, 1, 256)
         BRP_P1(4096, 2, 1024) BRP_P1(4096, 1, 512)
-        BRP_P1(384, 32, 256) BRP_P1(768, 16, 256) BRP_P1(1536, 8, 512) BRP_P1(3072, 4, 256)
+        BRP_P1(384, 32, 512) BRP_P1(768, 16, 512) BRP_P1(1536, 8, 512) BRP_P1(3072, 2, 256)
 #undef BRP_P1
         default: return false;
     }
@@ -670,6 +670,8 @@ bool with_pass2(uint32_t r, uint32_t s2, uint32_t t2, F&& f) {
         BRP_P2(64, 32, 256) BRP_P2(128, 16, 256) BRP_P2(256, 16, 512) BRP_P2(256, 8, 256)
         BRP_P2(512, 16, 1024) BRP_P2(512, 8, 512) BRP_P2(512, 4, 256) BRP_P2(512, 4, 512)
         BRP_P2(1024, 8, 1024) BRP_P2(1024, 4, 512) BRP_P2(1024, 2, 256) BRP_P2(2048, 4, 1024)
+        BRP_P2(192, 16, 256) BRP_P2(384, 8, 256) BRP_P2(768, 4, 256) BRP_P2(1536, 2, 256)
+        BRP_P2(384, 16, 512) BRP_P2(768, 8, 512)
 #undef BRP_P2
         default: return false;
     }
@@ -688,18 +690,26 @@ constexpr ShapeRow kShapes[] = {
     {1u << 18, 1024, 256, 16, 16, 1024, 512}, {1u << 19, 2048, 256, 8, 16, 1024, 512},
     {1u << 20, 4096, 256, 4, 16, 1024, 512},  {1u << 21, 4096, 512, 2, 4, 512, 256},
     {1u << 22, 4096, 1024, 4, 8, 1024, 1024},
-    {3u << 14, 384, 128, 32, 16, 256, 256},   {3u << 15, 768, 128, 16, 16, 256, 256},
-    {3u << 16, 768, 256, 16, 16, 256, 512},   {3u << 17, 1536, 256, 8, 16, 512, 512},
-    {3u << 18, 3072, 256, 4, 16, 256, 512},   {3u << 19, 3072, 512, 4, 16, 256, 1024},
-    {3u << 20, 3072, 1024, 4, 8, 256, 1024},  {3u << 21, 3072, 2048, 4, 4, 256, 1024},
+    // 3 * 2^k: the factor 3 in the pass-2 length R (24 elements per thread, radix 3, 8, ...),
+    // so pass 1 keeps its power-of-two row FFT at 16 elements per thread
+    {3u << 14, 256, 192, 16, 16, 256, 256},   {3u << 15, 256, 384, 16, 8, 256, 256},
+    {3u << 16, 512, 384, 16, 8, 512, 256},    {3u << 17, 1024, 384, 16, 8, 1024, 256},
+    {3u << 18, 2048, 384, 4, 8, 512, 256},    {3u << 19, 4096, 384, 2, 8, 512, 256},
+    {3u << 20, 4096, 768, 2, 4, 512, 256},    {3u << 21, 4096, 1536, 2, 2, 512, 256},
     // alternatives
+    {3u << 14, 384, 128, 32, 16, 512, 256},   {3u << 15, 768, 128, 16, 16, 512, 256},
+    {3u << 16, 768, 256, 16, 16, 512, 512},   {3u << 17, 1536, 256, 8, 16, 512, 512},
+    {3u << 18, 3072, 256, 2, 16, 256, 512},   {3u << 19, 3072, 512, 2, 16, 256, 1024},
+    {3u << 20, 3072, 1024, 2, 8, 256, 1024},  {3u << 21, 3072, 2048, 2, 4, 256, 1024},
+    {3u << 18, 4096, 192, 2, 16, 512, 256},   {3u << 19, 4096, 384, 2, 16, 512, 512},
+    {3u << 20, 4096, 768, 2, 8, 512, 512},
     {1u << 21, 4096, 512, 4, 16, 1024, 1024},
     {1u << 19, 1024, 512, 16, 16, 1024, 1024}, {1u << 19, 4096, 128, 4, 16, 1024, 256},
     {1u << 20, 2048, 512, 8, 16, 1024, 1024},  {1u << 20, 1024, 1024, 16, 8, 1024, 1024},
     {1u << 21, 2048, 1024, 8, 8, 1024, 1024},  {1u << 21, 1024, 2048, 16, 4, 1024, 1024},
     {1u << 22, 2048, 2048, 8, 4, 1024, 1024},
-    {3u << 18, 1536, 512, 8, 16, 512, 1024},   {3u << 18, 768, 1024, 16, 8, 256, 1024},
-    {3u << 19, 1536, 1024, 8, 8, 512, 1024},   {3u << 19, 768, 2048, 16, 4, 256, 1024},
+    {3u << 18, 1536, 512, 8, 16, 512, 1024},   {3u << 18, 768, 1024, 16, 8, 512, 1024},
+    {3u << 19, 1536, 1024, 8, 8, 512, 1024},   {3u << 19, 768, 2048, 16, 4, 512, 1024},
     {3u << 20, 1536, 2048, 8, 4, 512, 1024},
     // half-size blocks (512 threads, two blocks per CU)
     {1u << 19, 2048, 256, 4, 16, 512, 512},    {1u << 20, 4096, 256, 2, 16, 512, 512},

@@ -86,7 +86,7 @@ def test_fused_negative_phase(brp, oracle, wl, flags):
     assert np.array_equal(hot, snap.candidates)
 
 
-@pytest.mark.parametrize("n", [3 << 15, 1 << 17, 3 << 17])
+@pytest.mark.parametrize("n", [3 << 15, 1 << 17, 3 << 17, 3 << 18, 3 << 21, 3 << 22])
 def test_fused_lengths(brp, oracle, wl, n):
     w = wl.plain_full(seed=5, n=n, threshold_scale=0.3)
     p = w.templates[0]
@@ -138,6 +138,11 @@ ALTERNATIVE_SPLITS = [
     (1 << 23, 2048, 8, 4), (1 << 23, 4096, 2, 4),
     (3 << 19, 1536, 8, 16), (3 << 19, 768, 16, 8), (3 << 20, 1536, 8, 8), (3 << 20, 768, 16, 4),
     (3 << 21, 1536, 8, 4),
+    # 3 * 2^k with the factor 3 in pass 1 (C = 3 * 2^j, 24 elements per thread) or with the
+    # pass-2 length 3 * 2^j at 16 column pairs per block
+    (3 << 15, 384, 32, 16), (3 << 16, 768, 16, 16), (3 << 17, 768, 16, 16), (3 << 18, 1536, 8, 16),
+    (3 << 19, 3072, 2, 16), (3 << 20, 3072, 2, 16), (3 << 21, 3072, 2, 8), (3 << 22, 3072, 2, 4),
+    (3 << 19, 4096, 2, 16), (3 << 20, 4096, 2, 16), (3 << 21, 4096, 2, 8),
     (1 << 22, 4096, 1, 4), (1 << 22, 4096, 2, 4), (1 << 22, 4096, 1, 8), (1 << 22, 2048, 2, 2),
     (1 << 21, 4096, 1, 8), (1 << 22, 4096, 4, 16),
     # eight elements per thread (radix-8 plans): (N, C, S1, S2, T1, T2)
