@@ -568,26 +568,41 @@ __global__ __launch_bounds__(kHarmThreads) void k_harmonic(FusedArgs a, FusedSha
     }
     __syncthreads();
     // Phase B: level h, fundamental j owns bins [j 2^h - 2^(h-1), j 2^h + 2^(h-1)) ∩ [W, H);
-    // the tile owns the j with j 2^h in [i0, i0 + kTile), whose bins lie in its LDS rows.
+    // the tile owns the j with j 2^h in [i0, i0 + kTile), whose bins lie in its LDS rows
+    // (slots gi 2^h + 8 - 2^(h-1) .. for gi = j - j0).  Sums outside [W, H) are -inf, so the
+    // max over the whole interval is the max over its part in [W, H): all 2^h reads are
+    // issued together and reduced as a tree.  The value is the best sum if it passes the
+    // threshold, else the sum at the interval's first bin in [W, H), and 0 for an interval
+    // with none.  Levels 3 and 4 (128 and 64 fundamentals) go to different waves.
 #pragma unroll
     for (int h = 1; h <= 4; ++h) {
+        constexpr int kLane[5] = {0, 0, 0, 128, 0};  // first thread of the level's groups
         const int width = 1 << h;
         const int groups = kTile >> h;
         const int j0 = i0 >> h;
         const float thr = p.threshold[h];
-        for (int gi = t; gi < groups; gi += kHarmThreads) {
+        const float* row = sums[h - 1];
+#pragma unroll
+        for (int pass = 0; pass < (groups + kHarmThreads - 1) / kHarmThreads; ++pass) {
+            const int gi = t - kLane[h] + pass * kHarmThreads;
             const int j = j0 + gi;
-            if (j >= F) continue;
-            const int begin = max(W, j * width - width / 2);
-            const int end = min(H, j * width + width / 2);
+            if (gi < 0 || gi >= groups || j >= F) continue;
+            const int start = j * width - width / 2;  // first bin of the interval
+            const int lo = gi * width + 8 - width / 2;
+            float v[1 << 4];
+#pragma unroll
+            for (int d = 0; d < width; ++d) v[d] = row[sum_at(lo + d)];
+            const float at_start = v[0];
+#pragma unroll
+            for (int span = 1; span < width; span *= 2)
+#pragma unroll
+                for (int d = 0; d < width; d += 2 * span) v[d] = fmaxf(v[d], v[d + span]);
+            const int begin = max(W, start);
+            const int end = min(H, start + width);
             float value = 0.0f;
             if (begin < end) {
-                const float* row = sums[h - 1];
-                const int lo = begin - i0 + 8, hi = end - i0 + 8;
-                const float first = row[sum_at(lo)];
-                float best = first;
-                for (int d = lo + 1; d < hi; ++d) best = fmaxf(best, row[sum_at(d)]);
-                value = best > thr ? best : first;
+                const float first = begin == start ? at_start : row[sum_at(lo + begin - start)];
+                value = v[0] > thr ? v[0] : first;
             }
             if constexpr (SNAP) snap[size_t(h) * a.snap_f + j] = value;
             if (j >= W && value > thr) append_candidate(res, p.capacity, uint32_t(j), uint32_t(h), value);
