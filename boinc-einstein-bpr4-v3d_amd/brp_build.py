@@ -29,6 +29,8 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
              f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 HIP_SOURCES = ["plain.hip", "fused.hip", "preprocess.hip", "engine.hip"]
+CLI_SOURCES = ["brp_bench.cpp", "brp_workunit.cpp"]
+BIN = PKG / "bin"
 CXX_SOURCES = ["science.cpp"]
 
 
@@ -55,7 +57,23 @@ def build(verbose: bool = False) -> pathlib.Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objects, "-lpthread"])
     os.replace(tmp, LIB)
+    build_cli()
     return LIB
+
+
+def build_cli() -> list[pathlib.Path]:
+    """The command-line drivers (cli/*.cpp, the reference's brp_bench and brp_workunit)
+    into bin/, linked against the library with an rpath to it."""
+    BIN.mkdir(exist_ok=True)
+    out = []
+    for src in CLI_SOURCES:
+        exe = BIN / pathlib.Path(src).stem
+        tmp = exe.with_suffix(".tmp")
+        _run(["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", f"-I{PKG / 'cli'}",
+              str(PKG / "cli" / src), "-o", str(tmp), f"-L{PKG}", "-lbrp_hip", "-lz", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(tmp, exe)
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

@@ -110,6 +110,8 @@ def load() -> ctypes.CDLL:
         "brp_synthetic_dd": (i32, [u32, f64, ctypes.c_uint64, f64, f64, u32, f64, f64, f64, P(Orbit), u32, u32, f64,
                                    P(ctypes.c_uint8), P(u32)]),
         "brp_hip_preprocess": (i32, [P(ctypes.c_uint8), sz, u32, f64, u32, u32, f32, fp, fp, P(u32)]),
+        "brp_hip_preprocess_zap": (i32, [P(ctypes.c_uint8), sz, u32, f64, u32, u32, f32, P(f64), sz, f64, f64, fp,
+                                         fp, P(u32)]),
         "brp_uniform": (f64, [ctypes.c_uint64, ctypes.c_uint64]),
         "brp_hip_create": (i32, [P(Params), fp, sz, P(Options), P(ctypes.c_void_p)]),
         "brp_hip_destroy": (i32, [ctypes.c_void_p]),
@@ -202,14 +204,19 @@ def synthetic_dd(samples: int, dt: float, seed: int, mean: float = 7.5, sigma: f
 
 
 def preprocess(block: np.ndarray, samples: int, bits: int, scale: float, n: int, window: int = 1000,
-               rfi_ratio: float = 30.0):
-    """src/workload.cpp:29-95 on the device: returns (preprocessed, decoded, rfi bins replaced)."""
+               rfi_ratio: float = 30.0, zap=None, sample_us: float = 0.0, padding: float = 1.0):
+    """src/workload.cpp:29-95 on the device: returns (preprocessed, decoded, rfi bins replaced).
+    zap (optional): (lower, upper) Hz ranges replaced by noise after whitening, sized by
+    sample_us and padding as src/workload.cpp:76-85."""
     b = np.ascontiguousarray(block, dtype=np.uint8)
     out = np.empty(samples, dtype=np.float32)
     dec = np.empty(samples, dtype=np.float32)
     rfi = ctypes.c_uint32()
-    _check(load().brp_hip_preprocess(b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), samples, bits, scale, n,
-                                     window, rfi_ratio, _fptr(out), _fptr(dec), ctypes.byref(rfi)))
+    z = np.ascontiguousarray(np.asarray(zap if zap is not None else [], dtype=np.float64).reshape(-1))
+    _check(load().brp_hip_preprocess_zap(b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), samples, bits, scale, n,
+                                         window, rfi_ratio, z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         z.size // 2, sample_us, padding, _fptr(out), _fptr(dec),
+                                         ctypes.byref(rfi)))
     return out, dec, rfi.value
 
 

@@ -94,11 +94,14 @@ void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uin
 // bytes / 16 uint4 copies from src to dst (a multiple of 16 bytes).
 void copy16(hipStream_t st, const void* src, void* dst, size_t bytes);
 // ------------------------------------------------------------ preprocess --
-// Decode + whitening + RFI replacement of one observation (preprocess.hip); throws
-// std::runtime_error on an invalid median.  decoded (may be null) receives the decoded
-// samples, rfi_bins (may be null) the number of replaced bins.
+// Decode + whitening + RFI replacement + zap list of one observation (preprocess.hip);
+// throws std::runtime_error on an invalid median or zap range.  zap: zap_ranges pairs
+// (lower, upper) in Hz; sample_us and padding size the zap bins and noise.  decoded (may
+// be null) receives the decoded samples, rfi_bins (may be null) the number of replaced
+// bins.
 void preprocess_observation(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
-                            uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins);
+                            uint32_t window, float rfi_ratio, const double* zap, size_t zap_ranges, double sample_us,
+                            double padding, float* out, float* decoded, uint32_t* rfi_bins);
 
 // Copies {count, flags} and the first `keep` candidates of each of `nslots` slots into a
 // dense staging area of (keep + 1) x 16 bytes per slot.

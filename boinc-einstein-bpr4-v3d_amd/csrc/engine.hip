@@ -942,6 +942,13 @@ int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps) {
 
 int brp_hip_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
                        uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins) {
+    return brp_hip_preprocess_zap(bytes, samples, bits, scale, n, window, rfi_ratio, nullptr, 0, 0.0, 1.0, out,
+                                  decoded, rfi_bins);
+}
+
+int brp_hip_preprocess_zap(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                           uint32_t window, float rfi_ratio, const double* zap, size_t zap_ranges, double sample_us,
+                           double padding, float* out, float* decoded, uint32_t* rfi_bins) {
     return guarded([&] {
         require(bytes && out, "null argument");
         require(bits == 4 || bits == 8, "unsupported DD sample format");
@@ -951,10 +958,14 @@ int brp_hip_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, doub
                 "observation exceeds supported length");
         require(window && !(window & 1) && window <= n / 2 && window <= 3072, "unsupported whitening window");
         require(std::isfinite(rfi_ratio) && rfi_ratio >= 0, "invalid RFI ratio");
+        require(!zap_ranges || (zap && std::isfinite(sample_us) && sample_us > 0 && std::isfinite(padding) &&
+                                padding >= 1 && padding <= 10),
+                "invalid zap arguments");
         int ndev = 0;
         HIPC(hipGetDeviceCount(&ndev));
         require(ndev > 0, "no HIP device");
-        brp_engine::preprocess_observation(bytes, samples, bits, scale, n, window, rfi_ratio, out, decoded, rfi_bins);
+        brp_engine::preprocess_observation(bytes, samples, bits, scale, n, window, rfi_ratio, zap, zap_ranges,
+                                           sample_us, padding, out, decoded, rfi_bins);
     });
 }
 

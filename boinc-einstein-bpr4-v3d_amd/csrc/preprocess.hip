@@ -15,6 +15,15 @@
 //   RFI        new behaviour (the reference zaps a given list with GSL noise instead,
 //              src/workload.cpp:76-85): a bin with P[k] > ratio * m[k - w/2] is first
 //              scaled to |X[k]|^2 = m[k - w/2], keeping its phase
+//   zap        after whitening, the bins of each listed range [lower, upper] Hz,
+//              uint32(f * T + 0.5) with T = n * sample_us * 1e-6, inclusive, become
+//              sigma * (g1, g2) with sigma = sqrt(2)/2 * sqrt(padding)
+//              (src/workload.cpp:76-85); bins outside [w/2, n/2 + 1 - w/2) stay zero
+//              (src/workload.cpp:86).  g1, g2: Box-Muller from the counter-based uniform
+//              of this library (brp_uniform, counter 2k and 2k + 1 for bin k), seeded by
+//              the bit pattern of the first decoded sample as the reference seeds its
+//              generator; the reference draws from GSL's taus2 ziggurat instead, which
+//              is not available, so the zapped values are this build's own.
 //   inverse    out[j] = c2r_n(X)[j] / sqrt(n) for j < samples (src/workload.cpp:87-92)
 //
 // The transforms are the plain engine's Stockham passes: the forward one packs pairs
@@ -22,6 +31,7 @@
 // E = (X_k + conj X_{M-k}) / 2, O = W_n^{-k} (X_k - conj X_{M-k}) / 2, Z = E + i O, and
 // takes conj(FFT(conj Z)) / M.
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -134,6 +144,16 @@ __global__ __launch_bounds__(kBlock) void k_whiten(float2* __restrict__ X, const
     X[k] = make_float2(x.x * factor, x.y * factor);
 }
 
+// Zapped bins (bin, value) in place, for bins inside the whitened band.
+__global__ __launch_bounds__(kBlock) void k_zap(float2* __restrict__ X, uint32_t bins, uint32_t w,
+                                                const uint32_t* __restrict__ bin, const float2* __restrict__ value,
+                                                uint32_t count) {
+    const uint32_t z = blockIdx.x * kBlock + threadIdx.x;
+    if (z >= count) return;
+    const uint32_t k = bin[z];
+    if (k >= w / 2 && k < bins - w / 2) X[k] = value[z];
+}
+
 // conj(Z[k]) for the inverse of the real split, k < M (X[0] and X[M] are zero after
 // whitening, and the general formula covers them anyway).
 __global__ __launch_bounds__(kBlock) void k_inverse_split(const float2* __restrict__ X, const float2* __restrict__ twr,
@@ -189,8 +209,39 @@ void check(hipError_t e, const char* what) {
 namespace brp_engine {
 
 void preprocess_observation(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
-                            uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins) {
+                            uint32_t window, float rfi_ratio, const double* zap, size_t zap_ranges, double sample_us,
+                            double padding, float* out, float* decoded, uint32_t* rfi_bins) {
     const uint32_t m = n / 2, bins = m + 1;
+    // Zap list first (src/workload.cpp:76-85), so that a bad range fails before any
+    // device work: bins and their noise values, one entry per bin (a bin named by two
+    // ranges gets the same value either way).
+    std::vector<uint32_t> zap_bin;
+    std::vector<float2> zap_value;
+    if (zap_ranges) {
+        const double seconds = double(n) * sample_us * 1e-6;
+        const float first_sample = bits == 4 ? float(double(bytes[0] >> 4) / scale)
+                                             : float(double(int8_t(bytes[0])) / scale);
+        uint32_t seed32;
+        std::memcpy(&seed32, &first_sample, 4);
+        const double sigma = std::sqrt(2.0) / 2 * std::sqrt(padding);
+        std::vector<uint8_t> seen(bins, 0);
+        for (size_t r = 0; r < zap_ranges; ++r) {
+            const double lower = zap[2 * r], upper = zap[2 * r + 1];
+            if (!std::isfinite(lower + upper) || lower < 0 || upper < lower) throw std::runtime_error("invalid zap range");
+            const double first = lower * seconds + 0.5, last = upper * seconds + 0.5;
+            if (last >= double(bins)) throw std::runtime_error("zap exceeds Nyquist");
+            for (uint32_t k = uint32_t(first); k <= uint32_t(last); ++k) {
+                if (seen[k]) continue;
+                seen[k] = 1;
+                const double u1 = brp_uniform(seed32, 2ull * k), u2 = brp_uniform(seed32, 2ull * k + 1);
+                const double radius = std::sqrt(-2.0 * std::log(1.0 - u1));
+                const double angle = 6.283185307179586476925286766559 * u2;
+                zap_bin.push_back(k);
+                zap_value.push_back(make_float2(float(sigma * radius * std::cos(angle)),
+                                                float(sigma * radius * std::sin(angle))));
+            }
+        }
+    }
     DeviceBuffers d;
     hipStream_t st = nullptr;
     check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreateWithFlags");
@@ -229,6 +280,14 @@ void preprocess_observation(const uint8_t* bytes, size_t samples, uint32_t bits,
     k_spectrum_power<<<blocks(bins), kBlock, 0, st>>>(b.X, bins, P);
     k_running_median<<<(nmed + kMedianOut - 1) / kMedianOut, 1024, 0, st>>>(P, bins, window, med, counters);
     k_whiten<<<blocks(bins), kBlock, 0, st>>>(b.X, P, med, bins, window, rfi_ratio, counters + 1);
+    if (!zap_bin.empty()) {
+        const uint32_t nz = uint32_t(zap_bin.size());
+        uint32_t* dbin = d.get<uint32_t>(nz);
+        float2* dval = d.get<float2>(nz);
+        check(hipMemcpyAsync(dbin, zap_bin.data(), sizeof(uint32_t) * nz, hipMemcpyHostToDevice, st), "copy zap bins");
+        check(hipMemcpyAsync(dval, zap_value.data(), sizeof(float2) * nz, hipMemcpyHostToDevice, st), "copy zap values");
+        k_zap<<<blocks(nz), kBlock, 0, st>>>(b.X, bins, window, dbin, dval, nz);
+    }
     k_inverse_split<<<blocks(m), kBlock, 0, st>>>(b.X, b.tw_real, m, b.a);
     const float2* wv = plain_fft(st, b, true, true);
     k_unpack<<<blocks(m), kBlock, 0, st>>>(wv, m, uint32_t(samples), float(std::sqrt(double(n)) / double(m)), dout);

@@ -101,6 +101,14 @@ public:
         impl_.reset(h);
         templates_.assign(1, p);
     }
+    // Every backend option (brp_options: the reference's switches plus engine, group,
+    // split, lanes and the other extensions).
+    GPU(const Params& p, std::span<const float> input, const brp_options& o) : p_(p), batch_(o.batch) {
+        brp_hip* h = nullptr;
+        check(brp_hip_create(&p, input.data(), input.size(), &o, &h));
+        impl_.reset(h);
+        templates_.assign(1, p);
+    }
     GPU(const GPU&) = delete;
     GPU& operator=(const GPU&) = delete;
 
@@ -187,6 +195,19 @@ inline std::vector<float> preprocess(std::span<const uint8_t> block, size_t samp
                                      uint32_t* rfi_bins = nullptr) {
     std::vector<float> out(samples);
     check(brp_hip_preprocess(block.data(), samples, bits, scale, n, window, rfi_ratio, out.data(), nullptr, rfi_bins));
+    return out;
+}
+
+// The same with the reference's zap list (src/workload.cpp:76-86): (lower, upper) Hz
+// pairs flattened; decoded (may be null) receives the decoded samples.
+inline std::vector<float> preprocess(std::span<const uint8_t> block, size_t samples, uint32_t bits, double scale,
+                                     uint32_t n, uint32_t window, float rfi_ratio, std::span<const double> zap,
+                                     double sample_us, double padding, std::vector<float>* decoded = nullptr,
+                                     uint32_t* rfi_bins = nullptr) {
+    std::vector<float> out(samples);
+    if (decoded) decoded->resize(samples);
+    check(brp_hip_preprocess_zap(block.data(), samples, bits, scale, n, window, rfi_ratio, zap.data(), zap.size() / 2,
+                                 sample_us, padding, out.data(), decoded ? decoded->data() : nullptr, rfi_bins));
     return out;
 }
 

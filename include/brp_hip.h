@@ -175,6 +175,17 @@ double brp_uniform(uint64_t seed, uint64_t counter);
  * ("invalid whitening median", as src/workload.cpp:68). */
 int brp_hip_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
                        uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins);
+/* brp_hip_preprocess with the reference's zap list (src/workload.cpp:76-86): zap holds
+ * zap_ranges (lower, upper) pairs in Hz; after whitening, the bins uint32(f T + 0.5),
+ * T = n * sample_us * 1e-6, of each range (inclusive) are replaced by Gaussian noise of
+ * sigma sqrt(2)/2 * sqrt(padding) per component, except the band edges, which stay zero.
+ * The noise is this library's (Box-Muller from brp_uniform, seeded by the bit pattern of
+ * the first decoded sample, counters 2k and 2k + 1 for bin k), not GSL's taus2 ziggurat.
+ * Errors as the reference: "invalid zap range" (non-finite, lower < 0 or upper < lower)
+ * and "zap exceeds Nyquist" (last bin > n/2). */
+int brp_hip_preprocess_zap(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                           uint32_t window, float rfi_ratio, const double* zap, size_t zap_ranges, double sample_us,
+                           double padding, float* out, float* decoded, uint32_t* rfi_bins);
 
 /* ------------------------------------------------------------ device -------- */
 

@@ -482,8 +482,28 @@ static void cpu_template(const float* input, size_t count, const oracle_params& 
 // stated in oracle.h (the historical rngmed and the GSL zap noise are not available).
 int oracle_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
                       uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins) {
+    return oracle_preprocess_zap(bytes, samples, bits, scale, n, window, rfi_ratio, nullptr, 0, 0.0, 1.0, out, decoded,
+                                 rfi_bins);
+}
+
+// SplitMix64 (Steele, Lea and Flood, "Fast splittable pseudorandom number generators",
+// OOPSLA 2014) finaliser, and the counter-based uniform in [0, 1) the library defines
+// over it: mix(mix(seed + golden gamma) + counter * 0xD1B54A32D192ED03) >> 11, times 2^-53.
+static uint64_t splitmix_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double oracle_uniform(uint64_t seed, uint64_t counter) {
+    const uint64_t z = splitmix_mix(splitmix_mix(seed + 0x9E3779B97F4A7C15ull) + counter * 0xD1B54A32D192ED03ull);
+    return double(z >> 11) * 0x1.0p-53;
+}
+
+int oracle_preprocess_zap(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                          uint32_t window, float rfi_ratio, const double* zap, size_t zap_ranges, double sample_us,
+                          double padding, float* out, float* decoded, uint32_t* rfi_bins) {
     if (!bytes || !out || (bits != 4 && bits != 8) || samples > n || n % 2 || !window || (window & 1) ||
-        window > n / 2)
+        window > n / 2 || (zap_ranges && !zap))
         return fail("invalid preprocessing arguments");
     std::vector<double> time(n, 0.0);
     for (size_t i = 0; i < samples; ++i) {
@@ -521,6 +541,34 @@ int oracle_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, doubl
         x *= double(float(std::sqrt(0.69314718055994530942 / double(med))));
         full[k] = x;
         if (k < m) full[n - k] = std::conj(x);
+    }
+    // Zap list (src/workload.cpp:76-86): after whitening, each listed range's bins
+    // uint32(f T + 0.5) .. (inclusive) get Gaussian noise of sigma sqrt(2)/2 sqrt(padding)
+    // per component; the band edges stay zero.  Noise: Box-Muller over oracle_uniform
+    // (counters 2k, 2k + 1) seeded by the bit pattern of the first decoded sample.
+    if (zap_ranges) {
+        const double seconds = double(n) * sample_us * 1e-6;
+        const float first_sample = bits == 4 ? float(double(bytes[0] >> 4) / scale)
+                                             : float(double(int8_t(bytes[0])) / scale);
+        uint32_t seed32;
+        std::memcpy(&seed32, &first_sample, 4);
+        const double sigma = std::sqrt(2.0) / 2 * std::sqrt(padding);
+        for (size_t r = 0; r < zap_ranges; ++r) {
+            const double lower = zap[2 * r], upper = zap[2 * r + 1];
+            if (!std::isfinite(lower + upper) || lower < 0 || upper < lower) return fail("invalid zap range");
+            const double first = lower * seconds + 0.5, last = upper * seconds + 0.5;
+            if (last >= double(bins)) return fail("zap exceeds Nyquist");
+            for (uint32_t k = uint32_t(first); k <= uint32_t(last); ++k) {
+                if (k < window / 2 || k >= bins - window / 2) continue;
+                const double u1 = oracle_uniform(seed32, 2ull * k), u2 = oracle_uniform(seed32, 2ull * k + 1);
+                const double radius = std::sqrt(-2.0 * std::log(1.0 - u1));
+                const double angle = 6.283185307179586476925286766559 * u2;
+                const std::complex<double> x(float(sigma * radius * std::cos(angle)),
+                                             float(sigma * radius * std::sin(angle)));
+                full[k] = x;
+                if (k < m) full[n - k] = std::conj(x);
+            }
+        }
     }
     // Unnormalised inverse over the full Hermitian spectrum: y_j = Re FFT(conj X)_j.
     for (auto& c : full) c = std::conj(c);

@@ -107,6 +107,16 @@ double oracle_benchmark_cpu(const float* input, size_t count, const oracle_param
  * samples floats; rfi_bins (may be NULL): replaced bins. */
 int oracle_preprocess(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
                       uint32_t window, float rfi_ratio, float* out, float* decoded, uint32_t* rfi_bins);
+/* oracle_preprocess with the zap list of src/workload.cpp:76-86 (zap_ranges (lower, upper)
+ * Hz pairs, bins uint32(f n sample_us 1e-6 + 0.5), inclusive, replaced after whitening by
+ * Gaussian noise of sigma sqrt(2)/2 sqrt(padding) from oracle_uniform; band edges stay
+ * zero).  The reference draws GSL taus2 ziggurat noise, which is absent: the noise
+ * values are parity unpinned, the bins, sigma and edges are checked. */
+int oracle_preprocess_zap(const uint8_t* bytes, size_t samples, uint32_t bits, double scale, uint32_t n,
+                          uint32_t window, float rfi_ratio, const double* zap, size_t zap_ranges, double sample_us,
+                          double padding, float* out, float* decoded, uint32_t* rfi_bins);
+/* Counter-based uniform in [0, 1): SplitMix64 finaliser of seed and counter. */
+double oracle_uniform(uint64_t seed, uint64_t counter);
 
 /* The 65-entry (sin, cos) table used by the LUT sine, as 130 interleaved floats. */
 void oracle_lut_table(float* out);

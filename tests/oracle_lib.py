@@ -70,6 +70,9 @@ def load() -> ctypes.CDLL:
         "oracle_resample": (i32, [fp, sz, P(OParams), fp, P(ctypes.c_int32)]),
         "oracle_preprocess": (i32, [P(ctypes.c_uint8), sz, u32, ctypes.c_double, u32, u32, ctypes.c_float, fp, fp,
                                     P(u32)]),
+        "oracle_preprocess_zap": (i32, [P(ctypes.c_uint8), sz, u32, ctypes.c_double, u32, u32, ctypes.c_float,
+                                        dp, sz, ctypes.c_double, ctypes.c_double, fp, fp, P(u32)]),
+        "oracle_uniform": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64]),
         "oracle_rfft_f64": (i32, [dp, sz, dp]),
         "oracle_rfft_f32": (i32, [fp, sz, fp]),
         "oracle_power": (i32, [fp, sz, fp, dp]),
@@ -228,13 +231,20 @@ def benchmark_cpu(x: np.ndarray, templates, threads: int, reps: int) -> float:
 
 
 def preprocess(block: np.ndarray, samples: int, bits: int, scale: float, n: int, window: int = 1000,
-               rfi_ratio: float = 30.0):
-    """src/workload.cpp:29-95 restated in FP64: (preprocessed, decoded, rfi bins replaced)."""
+               rfi_ratio: float = 30.0, zap=None, sample_us: float = 0.0, padding: float = 1.0):
+    """src/workload.cpp:29-95 restated in FP64: (preprocessed, decoded, rfi bins replaced);
+    zap: optional (lower, upper) Hz ranges."""
     b = np.ascontiguousarray(block, dtype=np.uint8)
     out = np.empty(samples, dtype=np.float32)
     dec = np.empty(samples, dtype=np.float32)
     rfi = ctypes.c_uint32()
-    _check(load().oracle_preprocess(b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), samples, bits, scale, n,
-                                    window, rfi_ratio, _fp(out), _fp(dec), ctypes.byref(rfi)))
+    z = np.ascontiguousarray(np.asarray(zap if zap is not None else [], dtype=np.float64).reshape(-1))
+    _check(load().oracle_preprocess_zap(b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), samples, bits, scale, n,
+                                        window, rfi_ratio, z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        z.size // 2, sample_us, padding, _fp(out), _fp(dec), ctypes.byref(rfi)))
     return out, dec, rfi.value
+
+
+def uniform(seed: int, counter: int) -> float:
+    return load().oracle_uniform(seed, counter)
 

@@ -361,6 +361,25 @@ def test_preprocess_parity(brp, oracle, n, bits):
     assert rel <= 1e-5
 
 
+def test_preprocess_zap_parity(brp, oracle):
+    """The zap list on the device against the oracle: the same decoded samples and a
+    series within rel-L2 1e-5, with ranges inside the band, across its low edge and a
+    single bin, and the reference's error for a range past Nyquist."""
+    n, sample_us, padding = 1 << 16, 64.0, 1.25
+    block, _ = brp.synthetic_dd(n, 64e-6, 13, 7.5, 2.0, 4, 200.0, 0.05, 0.1, (3600.0, 0.05, 1.0), n, 10, 50.0)
+    seconds = n * sample_us * 1e-6
+    zap = [(37.0, 41.5), (0.0, 2.0), (1000.0, 1000.0), (1500.0, 1530.25)]
+    say(f"[gpu] preprocess zap N={n} ranges={zap}")
+    out, dec, rfi = brp.preprocess(block, n, 4, 1.5, n, 1000, 30.0, zap, sample_us, padding)
+    ref, rdec, rrfi = oracle.preprocess(block, n, 4, 1.5, n, 1000, 30.0, zap, sample_us, padding)
+    assert np.array_equal(dec, rdec) and rfi == rrfi
+    rel = float(np.linalg.norm(out.astype(np.float64) - ref) / np.linalg.norm(ref))
+    say({"rel_l2": rel, "seconds": seconds})
+    assert rel <= 1e-5
+    with pytest.raises(RuntimeError, match="Nyquist"):
+        brp.preprocess(block, n, 4, 1.5, n, 1000, 30.0, [(0.0, (n / 2 + 1) / seconds)], sample_us, padding)
+
+
 def test_uneven_workunit_pipeline(brp, oracle, wl):
     """The uneven workunit end to end at reduced sizes: 4-bit DD blocks of three lengths
     with pulsars and RFI, preprocessed on the device, then one mixed bank in one launch

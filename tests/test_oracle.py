@@ -304,7 +304,23 @@ def test_toplist_replacement_and_final(oracle):
     assert q.threshold[0] == np.float32(21.0)
 
 
-def _numpy_preprocess(block, samples, bits, scale, n, window, rfi_ratio):
+_MASK64 = (1 << 64) - 1
+
+
+def _splitmix_mix(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK64
+    return z ^ (z >> 31)
+
+
+def _uniform(seed, counter):
+    """SplitMix64 counter-based uniform, written from the published finaliser."""
+    z = _splitmix_mix((_splitmix_mix((seed + 0x9E3779B97F4A7C15) & _MASK64) +
+                       counter * 0xD1B54A32D192ED03) & _MASK64)
+    return (z >> 11) * 2.0 ** -53
+
+
+def _numpy_preprocess(block, samples, bits, scale, n, window, rfi_ratio, zap=(), sample_us=0.0, padding=1.0):
     if bits == 4:
         v = np.empty(samples, dtype=np.float64)
         v[0::2] = block >> 4
@@ -326,6 +342,16 @@ def _numpy_preprocess(block, samples, bits, scale, n, window, rfi_ratio):
             x = x * math.sqrt(float(m) / float(P[k]))
             rfi += 1
         Y[k] = x * float(np.float32(math.sqrt(math.log(2) / float(m))))
+    if len(zap):
+        seconds = n * sample_us * 1e-6
+        seed = int(np.array([dec[0]], dtype=np.float32).view(np.uint32)[0])
+        sigma = math.sqrt(2.0) / 2 * math.sqrt(padding)
+        for lower, upper in zap:
+            for k in range(int(lower * seconds + 0.5), int(upper * seconds + 0.5) + 1):
+                if window // 2 <= k < len(X) - window // 2:
+                    r = math.sqrt(-2.0 * math.log(1.0 - _uniform(seed, 2 * k)))
+                    a = 2 * math.pi * _uniform(seed, 2 * k + 1)
+                    Y[k] = complex(float(np.float32(sigma * r * math.cos(a))), float(np.float32(sigma * r * math.sin(a))))
     y = np.fft.irfft(Y, n) * n / math.sqrt(n)
     return y[:samples], dec, rfi
 
@@ -344,6 +370,43 @@ def test_preprocess_matches_numpy_restatement(oracle, brp, n, bits):
     # whitened noise: unit mean power per bin, so the series has unit variance (minus the
     # zeroed edge bins)
     assert 0.85 < float(np.var(out)) < 1.1
+
+
+def test_uniform_matches_library_and_published_finaliser(oracle, brp):
+    """The oracle's counter-based uniform (the zap noise source) equals the library's
+    and a restatement from the published SplitMix64 finaliser, including counters whose
+    products wrap."""
+    for seed, counter in [(0, 0), (1, 1), (0x3F800000, 12345), (2 ** 32 - 1, 2 ** 40 + 7), (7, 2 ** 63 - 1)]:
+        u = oracle.uniform(seed, counter)
+        assert u == brp.load().brp_uniform(seed, counter) == _uniform(seed, counter)
+        assert 0.0 <= u < 1.0
+
+
+def test_preprocess_zap_list(oracle, brp):
+    """The zap list (src/workload.cpp:76-86) against the numpy restatement: the listed
+    ranges' bins carry the noise, the band edges stay zero, and the reference's errors
+    for ranges past Nyquist and malformed ranges."""
+    n, window, sample_us, padding = 4096, 64, 64.0, 1.0
+    block, _ = brp.synthetic_dd(n, 64e-6, 5, 7.5, 2.0, 4, 200.0, 0.05, 0.1, (3600.0, 0.05, 1.0), n, 0, 50.0)
+    seconds = n * sample_us * 1e-6
+    # bins 40..50, one range inside the low edge (bins 0..4 < window / 2, zero) and one
+    # single bin
+    zap = [(40 / seconds, 50 / seconds), (0.0, 4.2 / seconds), (1000 / seconds, 1000.2 / seconds)]
+    out, dec, _ = oracle.preprocess(block, n, 4, 1.5, n, window, 0.0, zap, sample_us, padding)
+    ref, _, _ = _numpy_preprocess(block, n, 4, 1.5, n, window, 0.0, zap, sample_us, padding)
+    plain, _, _ = oracle.preprocess(block, n, 4, 1.5, n, window, 0.0)
+    rms = float(np.sqrt(np.mean(ref ** 2)))
+    assert np.max(np.abs(out - ref)) <= 1e-6 * rms
+    # Exactly the listed in-band bins changed: bins 40..50 and 1000 (the edge range 0..4
+    # is zero either way).
+    d = np.abs(np.fft.rfft(out.astype(np.float64) - plain.astype(np.float64)))
+    changed = set(np.nonzero(d > 1e-3 * d.max())[0].tolist())
+    assert changed == set(range(40, 51)) | {1000}
+    with pytest.raises(RuntimeError, match="Nyquist"):
+        oracle.preprocess(block, n, 4, 1.5, n, window, 0.0, [(0.0, (n / 2 + 1) / seconds)], sample_us, padding)
+    for bad in ([(-1.0, 2.0)], [(3.0, 2.0)], [(math.nan, 2.0)]):
+        with pytest.raises(RuntimeError, match="zap range"):
+            oracle.preprocess(block, n, 4, 1.5, n, window, 0.0, bad, sample_us, padding)
 
 
 def test_preprocess_rejects_bad_windows(oracle, brp):
