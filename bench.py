@@ -190,6 +190,22 @@ def main() -> int:
     gu.close()
     log(f"[bench] batched without kernel events: {untimed_tps:.1f} templates/s, toplist identical {untimed_identical}")
 
+    # The same bank with the toplist on the device (thresholds raised and retain run in the
+    # launch stream; candidates never leave the device), without kernel events.
+    gd = brp.GPU(w.templates[0], w.x, 256, 0, args.batch, packed=True, engine=brp.ENGINE_FUSED,
+                 graph=not args.no_graph, group=args.group, split_c=args.split_c, split_s1=args.split_s1,
+                 split_s2=args.split_s2, split_t1=args.split_t1, split_t2=args.split_t2, lanes=args.lanes,
+                 untimed=True)
+    gd.bank_search_device(w.templates, w.orbits, [brp.Toplist(100)])
+    td0 = time.perf_counter()
+    for i in range(args.steps):
+        top_d = brp.Toplist(100)
+        gd.bank_search_device(w.templates, w.orbits, [top_d])
+    device_top_tps = args.templates * args.steps / (time.perf_counter() - td0)
+    device_top_identical = [(r.bin, r.harmonics, r.period, r.power) for r in top_d.table()] == batched_rows
+    gd.close()
+    log(f"[bench] device toplist: {device_top_tps:.1f} templates/s, toplist identical {device_top_identical}")
+
     # ------------------------------------------------------------------ plain --
     plain_n = min(args.plain_templates, args.templates)
     plain_tps, identical = None, None
@@ -383,6 +399,8 @@ def main() -> int:
                    "lanes": args.lanes,
                    "graph": not args.no_graph},
         "autotune": tuned,
+        "device_toplist": {"value": round(device_top_tps, 3), "unit": "templates/s",
+                           "identical": device_top_identical},
         "untimed_templates_per_s": round(untimed_tps, 3),
         "untimed_toplist_identical": untimed_identical,
         "plain_templates_per_s": round(plain_tps, 3) if plain_tps else None,

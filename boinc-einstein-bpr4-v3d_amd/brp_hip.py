@@ -110,6 +110,9 @@ def load() -> ctypes.CDLL:
         "brp_synthetic_dd": (i32, [u32, f64, ctypes.c_uint64, f64, f64, u32, f64, f64, f64, P(Orbit), u32, u32, f64,
                                    P(ctypes.c_uint8), P(u32)]),
         "brp_hip_preprocess": (i32, [P(ctypes.c_uint8), sz, u32, f64, u32, u32, f32, fp, fp, P(u32)]),
+        "brp_hip_bank_search_device": (i32, [ctypes.c_void_p, P(Params), P(Orbit), sz, P(ctypes.c_void_p), sz,
+                                             P(u32), P(f64)]),
+        "brp_hip_toplist_retain_device": (i32, [P(Params), P(Orbit), sz, P(Candidate), P(u32), ctypes.c_void_p]),
         "brp_hip_preprocess_zap": (i32, [P(ctypes.c_uint8), sz, u32, f64, u32, u32, f32, P(f64), sz, f64, f64, fp,
                                          fp, P(u32)]),
         "brp_uniform": (f64, [ctypes.c_uint64, ctypes.c_uint64]),
@@ -301,6 +304,16 @@ class Toplist:
         return list(out)
 
 
+def toplist_retain_device(top: "Toplist", templates, orbits, candidate_lists) -> None:
+    """Toplist::retain of each template's candidate list on the device, in order."""
+    tarr = (Params * len(templates))(*templates)
+    oarr = (Orbit * len(orbits))(*[o if isinstance(o, Orbit) else Orbit(*o) for o in orbits])
+    flat = [c for lst in candidate_lists for c in lst]
+    carr = (Candidate * max(len(flat), 1))(*flat)
+    counts = (ctypes.c_uint32 * max(len(candidate_lists), 1))(*[len(lst) for lst in candidate_lists])
+    _check(load().brp_hip_toplist_retain_device(tarr, oarr, len(tarr), carr, counts, top.handle))
+
+
 def write_candidates(path: str, rows, seconds: float) -> None:
     arr = (BankRow * len(rows))(*rows)
     _check(load().brp_write_candidates(os.fsencode(path), arr, len(rows), seconds))
@@ -437,6 +450,18 @@ class GPU:
         gpu_ms = ctypes.c_double()
         _check(load().brp_hip_bank_search_templates(self._h, tarr, oarr, len(tarr), harr, len(tops), counts,
                                                     ctypes.byref(gpu_ms)))
+        return np.frombuffer(counts, dtype=np.uint32).copy(), gpu_ms.value
+
+    def bank_search_device(self, templates, orbits, tops):
+        """bank_search_templates with the toplists retained on the device; returns
+        (counts, gpu_ms) and leaves each toplist as the host loop would."""
+        tarr = (Params * len(templates))(*templates)
+        oarr = (Orbit * len(orbits))(*[o if isinstance(o, Orbit) else Orbit(*o) for o in orbits])
+        harr = (ctypes.c_void_p * len(tops))(*[tp.handle for tp in tops])
+        counts = (ctypes.c_uint32 * len(tarr))()
+        gpu_ms = ctypes.c_double()
+        _check(load().brp_hip_bank_search_device(self._h, tarr, oarr, len(tarr), harr, len(tops), counts,
+                                                 ctypes.byref(gpu_ms)))
         return np.frombuffer(counts, dtype=np.uint32).copy(), gpu_ms.value
 
     def bank_search(self, orbits, samples: int, sample_us: float, top: Toplist, f0: float = 400,

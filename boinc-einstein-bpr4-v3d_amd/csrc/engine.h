@@ -93,6 +93,25 @@ void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a);
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles);
 // bytes / 16 uint4 copies from src to dst (a multiple of 16 bytes).
 void copy16(hipStream_t st, const void* src, void* dst, size_t bytes);
+// --------------------------------------------------------- device toplist --
+// One toplist row on the device (the host brp_bank_row's fields as the retain writes them:
+// float values, significance 0).  Tables are [toplist][level][row].
+struct TopRow {
+    float power, period, tau, psi;
+    uint32_t bin, harmonics;
+};
+constexpr uint32_t kDeviceToplistMaxRows = 2048;
+// p.threshold[h] = max(p.threshold[h], lowest[table * 5 + h]) for `count` slots
+// (toplist.hip); lowest holds each level's last-row power as a retain recorded it.
+void toplist_thresholds(hipStream_t st, brp_params* params, uint32_t count, const uint32_t* slot_table,
+                        const float* lowest);
+// Toplist::retain of `count` result slots in slot order into the device tables, then the
+// last-row power of each level into lowest[table * 5 + h]; error bits 1 (slot invalid or
+// overflowed), 2 (invalid candidate), 4 (too many candidates of a level).
+void toplist_retain(hipStream_t st, const uint32_t* results, uint32_t slot_words, const brp_params* params,
+                    uint32_t count, const uint32_t* slot_table, const float4* slot_orbit, uint32_t tables,
+                    TopRow* rows_dev, uint32_t rows, float* lowest, uint32_t* error);
+
 // ------------------------------------------------------------ preprocess --
 // Decode + whitening + RFI replacement + zap list of one observation (preprocess.hip);
 // throws std::runtime_error on an invalid median or zap range.  zap: zap_ranges pairs

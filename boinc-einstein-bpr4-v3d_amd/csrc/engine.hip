@@ -20,6 +20,7 @@
 #include "engine.h"
 #include "errors.h"
 #include "lut.h"
+#include "toplist_state.h"
 
 namespace {
 
@@ -150,6 +151,25 @@ struct brp_hip {
     uint64_t total_templates = 0;       // templates those covered
     uint32_t timed_launches[2] = {0, 0}, timed_templates[2] = {0, 0};  // of the last run of each half
 
+    // Device toplist (brp_hip_bank_search_device): while `device_top` is set, launch()
+    // raises each slot's thresholds before the kernels and runs the retain after them on
+    // top_stream, where it overlaps the next batch.  A batch's thresholds come from the last
+    // rows the retain of the previous batch on the same half recorded (dtop_lowest[half]),
+    // as the host loop freezes them, so they do not depend on timing.  Per slot: its table
+    // and its orbit (period, tau, psi as float).
+    bool device_top = false;
+    uint32_t dtop_rows = 0, dtop_tables = 0, dtop_capacity = 0;  // capacity: tables allocated
+    brp_engine::TopRow* dtop = nullptr;
+    float* dtop_lowest[2] = {nullptr, nullptr};  // per half: tables x 5
+    uint32_t* dtop_error = nullptr;
+    hipStream_t top_stream = nullptr;
+    hipEvent_t kernels_done[2] = {nullptr, nullptr}, retained[2] = {nullptr, nullptr};
+    bool retain_pending[2] = {false, false};
+    uint32_t* slot_table_dev = nullptr;
+    float4* slot_orbit_dev = nullptr;
+    uint32_t* slot_table_host = nullptr;  // pinned, 2 * batch
+    float4* slot_orbit_host = nullptr;    // pinned, 2 * batch
+
     size_t slot_base(int half) const { return size_t(half) * opt.batch; }
 
     template <class T>
@@ -187,12 +207,18 @@ struct brp_hip {
     }
     ~brp_hip() {
         if (stream) hipStreamSynchronize(stream);
+        if (top_stream) hipStreamSynchronize(top_stream);
         for (auto g : graph_exec) if (g) hipGraphExecDestroy(g);
         for (auto& list : events) for (hipEvent_t e : list) hipEventDestroy(e);
         for (hipEvent_t e : done) if (e) hipEventDestroy(e);
         for (auto& a : allocations) hipFree(a.first);
         if (staging_host) hipHostFree(staging_host);
         if (params_host) hipHostFree(params_host);
+        if (slot_table_host) hipHostFree(slot_table_host);
+        if (slot_orbit_host) hipHostFree(slot_orbit_host);
+        for (hipEvent_t e : kernels_done) if (e) hipEventDestroy(e);
+        for (hipEvent_t e : retained) if (e) hipEventDestroy(e);
+        if (top_stream) hipStreamDestroy(top_stream);
         if (fork) hipEventDestroy(fork);
         if (join) hipEventDestroy(join);
         if (side) hipStreamDestroy(side);
@@ -580,8 +606,18 @@ void launch(brp_hip& h, int half) {
         h.params_host[base + i].result_slot = uint32_t(base + i);
     }
     const auto submit0 = std::chrono::steady_clock::now();
+    // The previous retain of this half reads its slots and parameters: wait for it.
+    if (h.device_top && h.retain_pending[half]) HIPC(hipStreamWaitEvent(h.stream, h.retained[half], 0));
     HIPC(hipMemcpyAsync(h.params_dev + base, h.params_host + base, sizeof(brp_params) * count,
                         hipMemcpyHostToDevice, h.stream));
+    if (h.device_top) {
+        HIPC(hipMemcpyAsync(h.slot_table_dev + base, h.slot_table_host + base, sizeof(uint32_t) * count,
+                            hipMemcpyHostToDevice, h.stream));
+        HIPC(hipMemcpyAsync(h.slot_orbit_dev + base, h.slot_orbit_host + base, sizeof(float4) * count,
+                            hipMemcpyHostToDevice, h.stream));
+        brp_engine::toplist_thresholds(h.stream, h.params_dev + base, count, h.slot_table_dev + base,
+                                       h.dtop_lowest[half]);
+    }
     if (h.opt.engine == BRP_ENGINE_PLAIN) {
         require(half == 0, "the plain engine runs one batch at a time");
         if (h.opt.snapshot)
@@ -595,6 +631,16 @@ void launch(brp_hip& h, int half) {
             HIPC(hipMemsetAsync(h.fa.snap_X, 0, sizeof(float2) * size_t(count) * h.fa.snap_m1, h.stream));
         }
         run_fused(h, half);
+    }
+    if (h.device_top) {
+        // Retains run in batch order on top_stream, each after its batch's kernels.
+        HIPC(hipEventRecord(h.kernels_done[half], h.stream));
+        HIPC(hipStreamWaitEvent(h.top_stream, h.kernels_done[half], 0));
+        brp_engine::toplist_retain(h.top_stream, h.results + base * h.slot_words, h.slot_words, h.params_dev + base,
+                                   count, h.slot_table_dev + base, h.slot_orbit_dev + base, h.dtop_tables, h.dtop,
+                                   h.dtop_rows, h.dtop_lowest[half], h.dtop_error);
+        HIPC(hipEventRecord(h.retained[half], h.top_stream));
+        h.retain_pending[half] = true;
     }
     h.submit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - submit0).count();
     HIPC(hipMemcpyAsync(h.staging_host + base * (h.keep + 1) * 4, h.staging_dev + base * (h.keep + 1) * 4,
@@ -1049,9 +1095,227 @@ void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit*
     if (gpu_ms) *gpu_ms = total_gpu;
 }
 
+// Device toplist state for `tables` toplists of `rows` rows (allocated once, grown as needed).
+void ensure_device_toplist(brp_hip& h, uint32_t tables, uint32_t rows) {
+    const size_t slots = 2 * size_t(h.opt.batch);
+    if (!h.slot_table_dev) {
+        h.slot_table_dev = h.alloc<uint32_t>(slots);
+        h.slot_orbit_dev = h.alloc<float4>(slots);
+        h.dtop_error = h.alloc<uint32_t>(1);
+        HIPC(hipHostMalloc(reinterpret_cast<void**>(&h.slot_table_host), sizeof(uint32_t) * slots, hipHostMallocDefault));
+        HIPC(hipHostMalloc(reinterpret_cast<void**>(&h.slot_orbit_host), sizeof(float4) * slots, hipHostMallocDefault));
+        HIPC(hipStreamCreateWithFlags(&h.top_stream, hipStreamNonBlocking));
+        for (int k = 0; k < 2; ++k) {
+            HIPC(hipEventCreateWithFlags(&h.kernels_done[k], hipEventDisableTiming));
+            HIPC(hipEventCreateWithFlags(&h.retained[k], hipEventDisableTiming));
+        }
+    }
+    const uint32_t need = tables * rows;
+    if (need > h.dtop_capacity) {
+        h.release(h.dtop);
+        h.release(h.dtop_lowest[0]);
+        h.release(h.dtop_lowest[1]);
+        h.dtop = h.alloc<brp_engine::TopRow>(5 * size_t(need));
+        h.dtop_lowest[0] = h.alloc<float>(5 * size_t(tables));
+        h.dtop_lowest[1] = h.alloc<float>(5 * size_t(tables));
+        h.dtop_capacity = need;
+    }
+    h.dtop_tables = tables;
+    h.dtop_rows = rows;
+    h.retain_pending[0] = h.retain_pending[1] = false;
+}
+
+void upload_toplists(brp_hip& h, brp_toplist* const* tops, size_t ntops) {
+    std::vector<brp_engine::TopRow> rows(5 * size_t(h.dtop_rows) * ntops);
+    std::vector<float> lowest(5 * ntops);
+    for (size_t k = 0; k < ntops; ++k) {
+        for (size_t r = 0; r < 5 * size_t(h.dtop_rows); ++r) {
+            const brp_bank_row& b = tops[k]->table[r];
+            rows[k * 5 * h.dtop_rows + r] =
+                brp_engine::TopRow{float(b.power), float(b.period), float(b.tau), float(b.psi), b.bin, b.harmonics};
+        }
+        for (size_t lv = 0; lv < 5; ++lv) lowest[k * 5 + lv] = float(tops[k]->table[(lv + 1) * h.dtop_rows - 1].power);
+    }
+    HIPC(hipStreamSynchronize(h.top_stream));
+    HIPC(hipMemcpyAsync(h.dtop, rows.data(), sizeof(brp_engine::TopRow) * rows.size(), hipMemcpyHostToDevice, h.stream));
+    for (int half = 0; half < 2; ++half)
+        HIPC(hipMemcpyAsync(h.dtop_lowest[half], lowest.data(), sizeof(float) * lowest.size(), hipMemcpyHostToDevice,
+                            h.stream));
+    HIPC(hipMemsetAsync(h.dtop_error, 0, sizeof(uint32_t), h.stream));
+    HIPC(hipStreamSynchronize(h.stream));
+}
+
+void download_toplists(brp_hip& h, brp_toplist* const* tops, size_t ntops) {
+    std::vector<brp_engine::TopRow> rows(5 * size_t(h.dtop_rows) * ntops);
+    uint32_t error = 0;
+    HIPC(hipStreamSynchronize(h.top_stream));
+    HIPC(hipMemcpyAsync(rows.data(), h.dtop, sizeof(brp_engine::TopRow) * rows.size(), hipMemcpyDeviceToHost, h.stream));
+    HIPC(hipMemcpyAsync(&error, h.dtop_error, sizeof(uint32_t), hipMemcpyDeviceToHost, h.stream));
+    HIPC(hipStreamSynchronize(h.stream));
+    if (error & 1u) throw Failure(BRP_E_OVERFLOW, "GPU invalid resample index or candidate overflow; no valid result");
+    if (error & 2u) throw Failure(BRP_E_INVALID, "invalid compact candidate");
+    if (error & 4u) throw Failure(BRP_E_RUNTIME, "device toplist: more than 4096 candidates of one level in one template");
+    for (size_t k = 0; k < ntops; ++k)
+        for (size_t r = 0; r < 5 * size_t(h.dtop_rows); ++r) {
+            const brp_engine::TopRow& d = rows[k * 5 * h.dtop_rows + r];
+            tops[k]->table[r] = brp_bank_row{double(d.power), double(d.period), double(d.tau), double(d.psi), 0.0,
+                                             d.harmonics, d.bin};
+        }
+}
+
+// The bank loop of bank_loop with the toplists on the device: the same launch sequence
+// and two batches in flight, with thresholds raised and retain() run in the launch stream,
+// so the host only collects candidate counts.  The toplists equal bank_loop's.
+void bank_loop_device(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit* orbits,
+                      brp_toplist* const* tops, size_t ntops, uint32_t* cand_count, double* gpu_ms) {
+    require(h.opt.engine == BRP_ENGINE_FUSED && !h.opt.snapshot, "device toplist: fused engine without snapshots");
+    require(ntops > 0, "no toplist");
+    const uint32_t rows = tops[0] ? tops[0]->rows : 0;
+    for (size_t k = 0; k < ntops; ++k)
+        require(tops[k] && tops[k]->rows == rows, "device toplist: every toplist needs the same number of rows");
+    require(rows >= 1 && rows <= brp_engine::kDeviceToplistMaxRows, "device toplist: at most 2048 rows per level");
+    const size_t count = base.size();
+    std::vector<uint32_t> ser(count, 0);
+    for (size_t i = 0; i < count; ++i) {
+        check_template(h, base[i]);
+        ser[i] = series_of(h, base[i]);
+        require(ser[i] < ntops, "no toplist for a template's series");
+    }
+    std::vector<size_t> seq(count);
+    for (size_t i = 0; i < count; ++i) seq[i] = i;
+    std::stable_sort(seq.begin(), seq.end(), [&](size_t x, size_t y) { return ser[x] < ser[y]; });
+    ensure_device_toplist(h, uint32_t(ntops), rows);
+    upload_toplists(h, tops, ntops);
+    struct Off {
+        brp_hip& h;
+        ~Off() {
+            if (h.top_stream) hipStreamSynchronize(h.top_stream);
+            h.device_top = false;
+            h.retain_pending[0] = h.retain_pending[1] = false;
+        }
+    } off{h};
+    h.device_top = true;
+    const size_t batch = h.opt.batch;
+    const size_t batches = (count + batch - 1) / batch;
+    auto enqueue = [&](size_t b) {
+        const int half = int(b & 1);
+        const size_t first = b * batch, n = std::min(batch, count - first), at = h.slot_base(half);
+        h.templates[half].resize(n);
+        for (size_t k = 0; k < n; ++k) {
+            const size_t i = seq[first + k];
+            h.templates[half][k] = base[i];
+            h.slot_table_host[at + k] = ser[i];
+            h.slot_orbit_host[at + k] =
+                make_float4(float(orbits[i].period), float(orbits[i].tau), float(orbits[i].psi), 0.0f);
+        }
+        launch(h, half);
+    };
+    double total_gpu = 0;
+    if (batches) enqueue(0);
+    for (size_t b = 0; b < batches; ++b) {
+        const int half = int(b & 1);
+        if (b + 1 < batches) enqueue(b + 1);
+        finish(h, half);
+        for (int s = 0; s < 4; ++s) total_gpu += h.kernel_ms[half][s];
+        const size_t first = b * batch, n = h.templates[half].size();
+        for (size_t k = 0; k < n; ++k) {
+            const uint32_t* rec = h.staging_host + (h.slot_base(half) + k) * (h.keep + 1) * 4;
+            if (rec[1] || rec[0] > h.templates[half][k].capacity)
+                throw Failure(BRP_E_OVERFLOW, "GPU invalid resample index or candidate overflow; no valid result");
+            if (cand_count) cand_count[seq[first + k]] = rec[0];
+        }
+    }
+    download_toplists(h, tops, ntops);
+    if (gpu_ms) *gpu_ms = total_gpu;
+}
+
 }  // namespace
 
 extern "C" {
+
+int brp_hip_bank_search_device(brp_hip* h, const brp_params* templates, const brp_orbit* orbits, size_t count,
+                               brp_toplist* const* tops, size_t ntops, uint32_t* cand_count, double* gpu_ms) {
+    return guarded([&] {
+        require(h && templates && orbits && tops, "null argument");
+        std::vector<brp_params> base(templates, templates + count);
+        bank_loop_device(*h, base, orbits, tops, ntops, cand_count, gpu_ms);
+    });
+}
+
+int brp_hip_toplist_retain_device(const brp_params* templates, const brp_orbit* orbits, size_t count,
+                                  const brp_candidate* candidates, const uint32_t* cand_count, brp_toplist* top) {
+    return guarded([&] {
+        require(templates && orbits && top && (count == 0 || cand_count), "null argument");
+        require(top->rows >= 1 && top->rows <= brp_engine::kDeviceToplistMaxRows,
+                "device toplist: at most 2048 rows per level");
+        int ndev = 0;
+        HIPC(hipGetDeviceCount(&ndev));
+        require(ndev > 0, "no HIP device");
+        // Result slots in the engine's layout: {count, flags, 0, 0} then the candidates.
+        uint32_t widest = 0;
+        size_t total = 0;
+        for (size_t i = 0; i < count; ++i) {
+            widest = std::max(widest, cand_count[i]);
+            total += cand_count[i];
+        }
+        require(!total || candidates, "null argument");
+        const uint32_t words = 4 + 4 * widest;
+        std::vector<uint32_t> slots(size_t(count) * words, 0u);
+        std::vector<brp_params> params(templates, templates + count);
+        std::vector<float4> orbit(count);
+        size_t next = 0;
+        for (size_t i = 0; i < count; ++i) {
+            uint32_t* rec = slots.data() + i * words;
+            rec[0] = cand_count[i];
+            params[i].capacity = std::max(params[i].capacity, cand_count[i]);
+            std::memcpy(rec + 4, candidates + next, sizeof(brp_candidate) * cand_count[i]);
+            next += cand_count[i];
+            orbit[i] = make_float4(float(orbits[i].period), float(orbits[i].tau), float(orbits[i].psi), 0.0f);
+        }
+        std::vector<uint32_t> table(count, 0u);
+        std::vector<brp_engine::TopRow> rows(5 * size_t(top->rows));
+        for (size_t r = 0; r < rows.size(); ++r) {
+            const brp_bank_row& b = top->table[r];
+            rows[r] = brp_engine::TopRow{float(b.power), float(b.period), float(b.tau), float(b.psi), b.bin, b.harmonics};
+        }
+        struct Buffers {
+            std::vector<void*> held;
+            void* get(size_t bytes) {
+                void* p = nullptr;
+                HIPC(hipMalloc(&p, std::max<size_t>(bytes, 4)));
+                held.push_back(p);
+                return p;
+            }
+            ~Buffers() { for (void* p : held) hipFree(p); }
+        } d;
+        auto* dslots = static_cast<uint32_t*>(d.get(sizeof(uint32_t) * slots.size()));
+        auto* dparams = static_cast<brp_params*>(d.get(sizeof(brp_params) * params.size()));
+        auto* dtable = static_cast<uint32_t*>(d.get(sizeof(uint32_t) * table.size()));
+        auto* dorbit = static_cast<float4*>(d.get(sizeof(float4) * orbit.size()));
+        auto* drows = static_cast<brp_engine::TopRow*>(d.get(sizeof(brp_engine::TopRow) * rows.size()));
+        auto* derror = static_cast<uint32_t*>(d.get(sizeof(uint32_t)));
+        auto* dlowest = static_cast<float*>(d.get(sizeof(float) * 5));
+        HIPC(hipMemcpy(dslots, slots.data(), sizeof(uint32_t) * slots.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(dparams, params.data(), sizeof(brp_params) * params.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(dtable, table.data(), sizeof(uint32_t) * table.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(dorbit, orbit.data(), sizeof(float4) * orbit.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(drows, rows.data(), sizeof(brp_engine::TopRow) * rows.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemset(derror, 0, sizeof(uint32_t)));
+        brp_engine::toplist_retain(nullptr, dslots, words, dparams, uint32_t(count), dtable, dorbit, 1, drows,
+                                   top->rows, dlowest, derror);
+        HIPC(hipGetLastError());
+        uint32_t error = 0;
+        HIPC(hipMemcpy(rows.data(), drows, sizeof(brp_engine::TopRow) * rows.size(), hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(&error, derror, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (error & 2u) throw Failure(BRP_E_INVALID, "invalid compact candidate");
+        if (error & 4u) throw Failure(BRP_E_RUNTIME, "device toplist: more than 4096 candidates of one level in one template");
+        for (size_t r = 0; r < rows.size(); ++r) {
+            const brp_engine::TopRow& v = rows[r];
+            top->table[r] = brp_bank_row{double(v.power), double(v.period), double(v.tau), double(v.psi), 0.0,
+                                         v.harmonics, v.bin};
+        }
+    });
+}
 
 int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint32_t samples, double sample_us,
                         double f0, double false_alarm, uint32_t window, brp_toplist* top, brp_params* templates_out,

@@ -8,6 +8,7 @@
 #include "brp_hip.h"
 #include "errors.h"
 #include "lut.h"
+#include "toplist_state.h"
 
 #include <algorithm>
 #include <cmath>
@@ -96,11 +97,6 @@ uint64_t mix64(uint64_t z) {
 }
 
 }  // namespace
-
-struct brp_toplist {
-    uint32_t rows = 100;
-    std::vector<brp_bank_row> table;  // 5 levels x rows, power-descending per level
-};
 
 extern "C" {
 

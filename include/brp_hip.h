@@ -258,6 +258,23 @@ int brp_hip_add_series(brp_hip* h, const brp_params* p, const float* input, size
 int brp_hip_bank_search_templates(brp_hip* h, const brp_params* templates, const brp_orbit* orbits, size_t count,
                                   brp_toplist* const* tops, size_t ntops, uint32_t* cand_count, double* gpu_ms);
 
+/* brp_hip_bank_search_templates with the toplists on the device (the next step after
+ * compaction: Toplist::thresholds / retain, src/workload.cpp:126-141, run in the launch
+ * stream after each batch).  The toplists are uploaded at the start and written back at
+ * the end; they equal brp_hip_bank_search_templates' row for row.  Fused engine without
+ * snapshots; every toplist needs the same number of rows, at most 2048 per level; at most
+ * 4096 candidates of one level per template.  Candidates stay on the device; cand_count
+ * (may be NULL) still receives each template's count. */
+int brp_hip_bank_search_device(brp_hip* h, const brp_params* templates, const brp_orbit* orbits, size_t count,
+                               brp_toplist* const* tops, size_t ntops, uint32_t* cand_count, double* gpu_ms);
+/* Toplist::retain (src/workload.cpp:129-141) of `count` templates on the device, for
+ * candidates produced elsewhere: cand_count[i] candidates of template i, concatenated in
+ * `candidates` (any order within a template), each template retained with its own
+ * parameters and orbit, in order.  Equals brp_toplist_thresholds + brp_toplist_retain per
+ * template.  Same limits as brp_hip_bank_search_device. */
+int brp_hip_toplist_retain_device(const brp_params* templates, const brp_orbit* orbits, size_t count,
+                                  const brp_candidate* candidates, const uint32_t* cand_count, brp_toplist* top);
+
 /* Bank search: the brp_workunit bank loop (src/workunit_main.cpp:77-136) for a bank of
  * `count` orbits against the handle's series, in batches of options.batch templates,
  * with thresholds frozen at each batch start and Toplist::retain replayed in template

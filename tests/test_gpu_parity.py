@@ -344,6 +344,75 @@ def test_bank_uneven_toplists(brp, oracle, wl, lanes):
         assert [r.bin for r in tops[k].final()] == [r.bin for r in ref.final()], n
 
 
+@pytest.mark.parametrize("ties", [False, True])
+def test_device_retain_equals_host_on_random_streams(brp, ties):
+    """Toplist::retain on the device (brp_hip_toplist_retain_device) against the host
+    retain on random candidate streams: repeated bins, tie-heavy powers, rows per level
+    from 3 to 1000 (the stable single move of equal powers included).  Within a template
+    the (level, bin) pairs are distinct, as the compaction produces them."""
+    rng = np.random.default_rng(40 + ties)
+    n = 1 << 20
+    for rows in (3, 7, 100, 1000):
+        host, dev = brp.Toplist(rows), brp.Toplist(rows)
+        templates, orbits, lists = [], [], []
+        for t in range(80):
+            orbit = (float(rng.uniform(660, 10000)), float(rng.uniform(0, 0.3)), float(rng.uniform(0, 6.28)))
+            p = brp.search_parameters(n, n, 64.0, orbit)
+            k = int(rng.integers(0, 40))
+            levels = rng.integers(0, 5, k)
+            bins = rng.integers(p.window, p.fundamental, k)
+            bins[: k // 3] = rng.integers(p.window, p.window + 5, k // 3)
+            if ties:
+                powers = np.float32([100.0 * int(rng.integers(1, 4)) for _ in levels])
+            else:
+                powers = np.float32([p.threshold[l] * rng.uniform(0.9, 3.0) for l in levels])
+            seen, cands = set(), []
+            for b, lv, pw in zip(bins, levels, powers):
+                if (int(lv), int(b)) not in seen:
+                    seen.add((int(lv), int(b)))
+                    cands.append(brp.Candidate(int(b), int(lv), float(pw), 0))
+            templates.append(p)
+            orbits.append(orbit)
+            lists.append(list(reversed(cands)))  # any order within a template
+            q = p.copy()
+            host.thresholds(q)
+            host.retain(sorted(cands, key=lambda c: (c.level, c.bin)), orbit, q)
+        say(f"[gpu] device retain rows={rows} ties={ties} templates=80")
+        brp.toplist_retain_device(dev, templates, orbits, lists)
+        mine = [(r.bin, r.power, r.period, r.tau, r.psi, r.harmonics) for r in dev.table()]
+        ref = [(r.bin, r.power, r.period, r.tau, r.psi, r.harmonics) for r in host.table()]
+        assert mine == ref, rows
+
+
+@pytest.mark.parametrize("rows,lanes", [(100, 1), (1000, 1), (100, 2)])
+def test_bank_search_device_toplist_equals_host(brp, wl, rows, lanes):
+    """The bank with its toplists on the device (thresholds raised in the launch stream,
+    retain overlapping the next batch on its own stream) against the host bank loop:
+    identical toplists for every series of an uneven bank, and identical candidate counts
+    (both freeze a batch's thresholds at the toplist two batches back)."""
+    u = wl.uneven(count=48, lengths=(1 << 18, 3 << 17, 1 << 19))
+    first = u.templates[0]
+    xs = {n: x for n, x, _ in u.series}
+    reg = [first.n] + [n for n, _, _ in u.series if n != first.n]
+    say(f"[gpu] bank_search_device 48 templates over {reg}, rows={rows}, lanes={lanes}")
+    out = {}
+    for mode in ("host", "device"):
+        tops = [brp.Toplist(rows) for _ in reg]
+        with brp.GPU(first, xs[first.n], 256, 0, 16, packed=True, engine=brp.ENGINE_FUSED, graph=True, group=4,
+                     lanes=lanes) as g:
+            for n in reg[1:]:
+                g.add_series(brp.search_parameters(n, n, wl.SAMPLE_US, wl.INJECTED), xs[n])
+            if mode == "host":
+                counts, _ = g.bank_search_templates(u.templates, u.orbits, tops)
+            else:
+                counts, _ = g.bank_search_device(u.templates, u.orbits, tops)
+        out[mode] = ([[(r.bin, r.power, r.period, r.tau, r.psi, r.harmonics) for r in t.table()] for t in tops],
+                     counts)
+    assert out["host"][0] == out["device"][0]
+    assert out["device"][1].sum() > 0
+    assert np.array_equal(out["host"][1], out["device"][1])
+
+
 @pytest.mark.parametrize("n,bits", [(1 << 16, 4), (3 << 15, 4), (1 << 16, 8)])
 def test_preprocess_parity(brp, oracle, n, bits):
     """Device decode + whitening + RFI replacement against the FP64 oracle: decoded
