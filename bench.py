@@ -68,7 +68,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--templates", type=int, default=512)
     ap.add_argument("--n", type=int, default=1 << 22)
-    ap.add_argument("--batch", type=int, default=64, help="templates per run (toplist thresholds frozen per run)")
+    ap.add_argument("--batch", type=int, default=128, help="templates per run (toplist thresholds frozen per run)")
     ap.add_argument("--group", type=int, default=8, help="templates per kernel launch (without --no-tune: tuned)")
     ap.add_argument("--split-c", type=int, default=0, help="pass-1 FFT length (without --no-tune: tuned)")
     ap.add_argument("--split-s1", type=int, default=0, help="rows per pass-1 block (without --no-tune: tuned)")
