@@ -220,3 +220,27 @@ def test_bench_driver(args):
         assert key in out, key
     assert out["valid"] and out["replay_reproducible"] and out["index_mismatches"] == 0
     assert len(out["wall_ms_samples"]) == 3 and all(math.isfinite(v) and v > 0 for v in out["wall_ms_samples"])
+
+
+@pytest.mark.gpu
+def test_workunit_full_size(brp, tmp_path):
+    """brp_workunit at the production size: a 4-bit DD file of 2^22 samples and a bank of
+    512 orbits; the results file equals the library pipeline's byte for byte."""
+    n = 1 << 22
+    block, _ = brp.synthetic_dd(n, 64e-6, 42, 7.5, 2.0, 4, 211.0, 0.05, 0.2, (3600.0, 0.05, 1.0), n, 10, 50.0)
+    dd = write_dd(tmp_path / "obs.bin4", block, n)
+    rows = [(3600.0, 0.05, 1.0)] + [(o.period, o.tau, o.psi) for o in brp.synthetic_bank(43, 511)]
+    bank = tmp_path / "bank.txt"
+    bank.write_text("".join(f"{p!r} {t!r} {s!r}\n" for p, t, s in rows))
+    zap = tmp_path / "zap.txt"
+    zap.write_text("50.0 50.2\n")
+    files = {"n": n, "block": block, "dd": dd, "bank": bank, "zap": zap, "rows": rows}
+    say("[gpu] brp_workunit N=2^22, 512 templates")
+    r = run(workunit_args(tmp_path, dd, bank, zap, ["--rfi-ratio", "30", "--batch", "128"]), timeout=300)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout)
+    say(summary)
+    assert summary["templates"] == 512 and summary["valid"] and summary["candidates"] > 0
+    expected, rfi = _toplist_file(brp, files, tmp_path, 30.0)
+    assert rfi == summary["rfi_bins"]
+    assert (tmp_path / "out.txt").read_text() == expected
