@@ -732,7 +732,7 @@ constexpr ShapeRow kShapes[] = {
     {1u << 21, 4096, 512, 4, 8, 1024, 512},    {1u << 21, 2048, 1024, 4, 4, 512, 512},
     {1u << 22, 4096, 1024, 2, 4, 512, 512},
     // quarter-size blocks (256 threads, four blocks per CU)
-    {1u << 21, 4096, 512, 1, 4, 256, 256},
+    {1u << 21, 4096, 512, 1, 4, 256, 256},     {1u << 21, 2048, 1024, 2, 4, 256, 512},
     {1u << 21, 4096, 512, 1, 8, 256, 512},     {1u << 21, 2048, 1024, 2, 2, 256, 256},
     {1u << 20, 4096, 256, 1, 8, 256, 256},
     // eight elements per thread (radix-8 plans, fewer registers, more waves per CU)

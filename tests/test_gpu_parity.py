@@ -147,7 +147,9 @@ ALTERNATIVE_SPLITS = [
     (1 << 21, 4096, 1, 8), (1 << 22, 4096, 4, 16),
     # eight elements per thread (radix-8 plans): (N, C, S1, S2, T1, T2)
     (1 << 22, 4096, 2, 4, 1024, 256), (1 << 22, 4096, 1, 4, 512, 256), (1 << 22, 4096, 2, 4, 512, 512),
-    (1 << 22, 4096, 1, 4, 512, 512)]
+    (1 << 22, 4096, 1, 4, 512, 512),
+    # 256-thread pass-1 blocks on 2048-point rows, 512-thread pass-2 blocks
+    (1 << 22, 2048, 2, 4, 256, 512)]
 
 
 @pytest.mark.parametrize("shape", ALTERNATIVE_SPLITS)
