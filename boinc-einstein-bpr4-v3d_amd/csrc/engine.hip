@@ -842,7 +842,8 @@ size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap) {
         s << "; engine=fused";
         for (const auto& ser : h->series)
             s << " N=" << ser.n << " C=" << ser.shape.c << " R=" << ser.shape.r << " S1=" << ser.shape.s1
-              << " S2=" << ser.shape.s2 << " T1=" << ser.shape.t1 << " T2=" << ser.shape.t2 << ";";
+              << " S2=" << ser.shape.s2 << " T1=" << ser.shape.t1 << " T2=" << ser.shape.t2
+              << " exchange=" << (ser.shape.mirror ? "mirror" : "plain") << ";";
         s << " group=" << h->group << " lanes=" << h->lanes;
     }
     else if (h)

@@ -163,6 +163,17 @@ __device__ __forceinline__ cf twiddle2(const float2* __restrict__ lo, const floa
     return mul(load(hi + (e >> 11)), load(lo + (e & 2047u)));
 }
 
+// Position of column k2 in a row of the exchange Y.  Plain layout: k2.  Mirror layout
+// (FusedShape::mirror): every column beside its mirror C - k2 (0 and C/2 first: 0, C/2, 1,
+// C-1, 2, C-2, ...), so the columns a pass-2 block pairs up form one contiguous run of each
+// row.  That halves the segments pass 2 reads when it pairs few columns per block, at the
+// price of half-filled store instructions in pass 1; the shape table turns it on for
+// shapes of at most 4 column pairs per pass-2 block, where it measured faster.
+__device__ __forceinline__ uint32_t exchange_pos(uint32_t k2, uint32_t C) {
+    const uint32_t h = C / 2;
+    return k2 == 0 ? 0u : (k2 == h ? 1u : (k2 < h ? 2u * k2 : 2u * (C - k2) + 1u));
+}
+
 // ------------------------------------------------------------------ pass 1 --
 // Resample + pack the E elements of thread t: element e (butterfly u = e / R0, input
 // q = e % R0) is the packed pair z[m] = (y[2m], y[2m+1]) with m = r0 + s + R c,
@@ -310,8 +321,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 
 
     // Y[r][k2] = v * W_M^{r k2} for k2 = j + q C/RL: W_M^{r j} times
     // W_M^{r q C/RL} = W_{R RL}^{r q} (r q < R RL, read from the W_{16 R} table), all of a
-    // thread's table reads issued together.
+    // thread's table reads issued together.  Column k2 is stored at exchange_pos(k2, C):
+    // in the mirror layout 2 k2 below C/2 (from lo) and 2 (C - k2) + 1 above (from hi),
+    // C/2 at 1.
     float2* Y = a.Y + size_t(g) * M;
+    const bool mirror = sh.mirror != 0;
 #pragma unroll
     for (int u = 0; u < E / RL; ++u) {
         const int b = t + T * u;
@@ -321,11 +335,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 
 #pragma unroll
         for (int q = 1; q < RL; ++q) pw[q] = load(a.tw_rq + __umul24(r, uint32_t(q * (16 / RL))));
         const cf base = twiddle2(a.twm_lo, a.twm_hi, __umul24(r, uint32_t(j)));
-        float2* row = Y + size_t(r) * C + j;
+        float2* row = Y + size_t(r) * C;
+        if (mirror) {
+            float2* lo = row + 2 * j;
+            float2* hi = row + (2 * C + 1 - 2 * j);
 #pragma unroll
-        for (int q = 0; q < RL; ++q) {
-            const cf w = q == 0 ? base : mul(base, pw[q]);
-            store(row + q * (C / RL), mul(v[u * RL + q], w));
+            for (int q = 0; q < RL; ++q) {
+                const cf w = q == 0 ? base : mul(base, pw[q]);
+                float2* dst = q < RL / 2 ? lo + 2 * q * (C / RL)
+                                         : ((q == RL / 2 && j == 0) ? row + 1 : hi - 2 * q * (C / RL));
+                store(dst, mul(v[u * RL + q], w));
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+                const cf w = q == 0 ? base : mul(base, pw[q]);
+                store(row + j + q * (C / RL), mul(v[u * RL + q], w));
+            }
         }
     }
 
@@ -380,8 +406,9 @@ k_pass2(FusedArgs a, FusedShape sh) {
         const int b = t + T * u;
         const int s = b % SS, j = b / SS;
         const uint32_t col = column(s);
+        const uint32_t pos = sh.mirror ? exchange_pos(col, C) : col;
 #pragma unroll
-        for (int q = 0; q < R0; ++q) v[u * R0 + q] = load(Y + size_t(j + q * (R / R0)) * C + col);
+        for (int q = 0; q < R0; ++q) v[u * R0 + q] = load(Y + size_t(j + q * (R / R0)) * C + pos);
     }
 
     // Pad mean from the pass-1 partial sums, in a fixed order (bit-reproducible), while
@@ -703,7 +730,7 @@ constexpr ShapeRow kShapes[] = {
     {1u << 14, 128, 128, 32, 16, 256, 256},   {1u << 15, 256, 128, 16, 16, 256, 256},
     {1u << 16, 256, 256, 16, 16, 256, 512},   {1u << 17, 512, 256, 16, 16, 512, 512},
     {1u << 18, 1024, 256, 16, 16, 1024, 512}, {1u << 19, 2048, 256, 8, 16, 1024, 512},
-    {1u << 20, 4096, 256, 4, 16, 1024, 512},  {1u << 21, 4096, 512, 2, 4, 512, 256},
+    {1u << 20, 4096, 256, 4, 16, 1024, 512},  {1u << 21, 2048, 1024, 2, 4, 256, 512},
     {1u << 22, 4096, 1024, 4, 8, 1024, 1024},
     // 3 * 2^k: the factor 3 in the pass-2 length R (24 elements per thread, radix 3, 8, ...),
     // so pass 1 keeps its power-of-two row FFT at 16 elements per thread
@@ -732,7 +759,7 @@ constexpr ShapeRow kShapes[] = {
     {1u << 21, 4096, 512, 4, 8, 1024, 512},    {1u << 21, 2048, 1024, 4, 4, 512, 512},
     {1u << 22, 4096, 1024, 2, 4, 512, 512},
     // quarter-size blocks (256 threads, four blocks per CU)
-    {1u << 21, 4096, 512, 1, 4, 256, 256},     {1u << 21, 2048, 1024, 2, 4, 256, 512},
+    {1u << 21, 4096, 512, 1, 4, 256, 256},     {1u << 21, 4096, 512, 2, 4, 512, 256},
     {1u << 21, 4096, 512, 1, 8, 256, 512},     {1u << 21, 2048, 1024, 2, 2, 256, 256},
     {1u << 20, 4096, 256, 1, 8, 256, 256},
     // eight elements per thread (radix-8 plans, fewer registers, more waves per CU)
@@ -756,6 +783,7 @@ bool shape_of(const ShapeRow& row, uint32_t n, FusedShape* out) {
     s.n = n; s.m = n / 2; s.c = row.c; s.r = row.r; s.s1 = row.s1; s.s2 = row.s2; s.t1 = row.t1; s.t2 = row.t2;
     s.blocks1 = row.r / row.s1;
     s.blocks2 = row.c / (2 * row.s2) + 1;
+    s.mirror = row.s2 <= 4;
     *out = s;
     return true;
 }

@@ -148,8 +148,8 @@ ALTERNATIVE_SPLITS = [
     # eight elements per thread (radix-8 plans): (N, C, S1, S2, T1, T2)
     (1 << 22, 4096, 2, 4, 1024, 256), (1 << 22, 4096, 1, 4, 512, 256), (1 << 22, 4096, 2, 4, 512, 512),
     (1 << 22, 4096, 1, 4, 512, 512),
-    # 256-thread pass-1 blocks on 2048-point rows, 512-thread pass-2 blocks
-    (1 << 22, 2048, 2, 4, 256, 512)]
+    # 4096-point rows in 512-thread blocks, 256-thread pass-2 blocks
+    (1 << 22, 4096, 2, 4, 512, 256)]
 
 
 @pytest.mark.parametrize("shape", ALTERNATIVE_SPLITS)
