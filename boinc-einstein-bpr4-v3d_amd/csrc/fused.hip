@@ -729,14 +729,14 @@ constexpr ShapeRow kShapes[] = {
     {1u << 12, 64, 64, 64, 32, 256, 256},     {1u << 13, 128, 64, 32, 32, 256, 256},
     {1u << 14, 128, 128, 32, 16, 256, 256},   {1u << 15, 256, 128, 16, 16, 256, 256},
     {1u << 16, 256, 256, 16, 16, 256, 512},   {1u << 17, 512, 256, 16, 16, 512, 512},
-    {1u << 18, 1024, 256, 16, 16, 1024, 512}, {1u << 19, 2048, 256, 8, 16, 1024, 512},
-    {1u << 20, 4096, 256, 4, 16, 1024, 512},  {1u << 21, 2048, 1024, 2, 4, 256, 512},
+    {1u << 18, 1024, 256, 16, 16, 1024, 512}, {1u << 19, 2048, 256, 2, 8, 256, 256},
+    {1u << 20, 2048, 512, 2, 4, 256, 256},    {1u << 21, 2048, 1024, 2, 4, 256, 512},
     {1u << 22, 4096, 1024, 4, 8, 1024, 1024},
     // 3 * 2^k: the factor 3 in the pass-2 length R (24 elements per thread, radix 3, 8, ...),
     // so pass 1 keeps its power-of-two row FFT at 16 elements per thread
     {3u << 14, 256, 192, 16, 16, 256, 256},   {3u << 15, 256, 384, 16, 8, 256, 256},
     {3u << 16, 512, 384, 16, 8, 512, 256},    {3u << 17, 1024, 384, 16, 8, 1024, 256},
-    {3u << 18, 2048, 384, 4, 8, 512, 256},    {3u << 19, 4096, 384, 2, 8, 512, 256},
+    {3u << 18, 2048, 384, 2, 8, 256, 256},    {3u << 19, 2048, 768, 2, 4, 256, 256},
     {3u << 20, 4096, 768, 2, 4, 512, 256},    {3u << 21, 4096, 1536, 2, 2, 512, 256},
     // alternatives
     {3u << 14, 384, 128, 32, 16, 512, 256},   {3u << 15, 768, 128, 16, 16, 512, 256},
@@ -760,6 +760,8 @@ constexpr ShapeRow kShapes[] = {
     {1u << 22, 4096, 1024, 2, 4, 512, 512},
     // quarter-size blocks (256 threads, four blocks per CU)
     {1u << 21, 4096, 512, 1, 4, 256, 256},     {1u << 21, 4096, 512, 2, 4, 512, 256},
+    {1u << 19, 2048, 256, 8, 16, 1024, 512},   {1u << 20, 4096, 256, 4, 16, 1024, 512},
+    {3u << 18, 2048, 384, 4, 8, 512, 256},     {3u << 19, 4096, 384, 2, 8, 512, 256},
     {1u << 21, 4096, 512, 1, 8, 256, 512},     {1u << 21, 2048, 1024, 2, 2, 256, 256},
     {1u << 20, 4096, 256, 1, 8, 256, 256},
     // eight elements per thread (radix-8 plans, fewer registers, more waves per CU)

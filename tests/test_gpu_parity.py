@@ -86,7 +86,7 @@ def test_fused_negative_phase(brp, oracle, wl, flags):
     assert np.array_equal(hot, snap.candidates)
 
 
-@pytest.mark.parametrize("n", [3 << 15, 1 << 17, 3 << 17, 3 << 18, 3 << 21, 3 << 22])
+@pytest.mark.parametrize("n", [3 << 15, 1 << 17, 3 << 17, 3 << 18, 3 << 19, 1 << 20, 1 << 21, 3 << 21, 3 << 22])
 def test_fused_lengths(brp, oracle, wl, n):
     w = wl.plain_full(seed=5, n=n, threshold_scale=0.3)
     p = w.templates[0]
@@ -149,7 +149,10 @@ ALTERNATIVE_SPLITS = [
     (1 << 22, 4096, 2, 4, 1024, 256), (1 << 22, 4096, 1, 4, 512, 256), (1 << 22, 4096, 2, 4, 512, 512),
     (1 << 22, 4096, 1, 4, 512, 512),
     # 4096-point rows in 512-thread blocks, 256-thread pass-2 blocks
-    (1 << 22, 4096, 2, 4, 512, 256)]
+    (1 << 22, 4096, 2, 4, 512, 256),
+    # the earlier defaults of 2^20, 2^21, 3 * 2^19 and 3 * 2^20
+    (1 << 20, 2048, 8, 16, 1024, 512), (1 << 21, 4096, 4, 16, 1024, 512), (3 << 19, 2048, 4, 8, 512, 256),
+    (3 << 20, 4096, 2, 8, 512, 256)]
 
 
 @pytest.mark.parametrize("shape", ALTERNATIVE_SPLITS)
