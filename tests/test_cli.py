@@ -146,7 +146,7 @@ def test_workunit_bank_errors_and_uncompressed_input(tmp_path, files):
 
 def _toplist_file(brp, files, tmp_path, rfi_ratio, engine="fused"):
     """The same workunit through the library: device preprocessing with the zap list,
-    brp_hip_bank_search in batches of 64 (the driver's defaults: workgroup 256, Stockham,
+    brp_hip_bank_search in batches of 64 (brp_workunit's defaults: workgroup 256, Stockham,
     radix 4), final rows written by brp_write_candidates."""
     n = files["n"]
     zap = [tuple(map(float, line.split())) for line in files["zap"].read_text().splitlines()]
