@@ -86,7 +86,8 @@ def test_fused_negative_phase(brp, oracle, wl, flags):
     assert np.array_equal(hot, snap.candidates)
 
 
-@pytest.mark.parametrize("n", [3 << 15, 1 << 17, 3 << 17, 3 << 18, 3 << 19, 1 << 20, 1 << 21, 3 << 21, 3 << 22])
+@pytest.mark.parametrize("n", [1 << 13, 1 << 14, 1 << 15, 3 << 15, 1 << 17, 3 << 17, 3 << 18, 3 << 19, 1 << 20, 1 << 21,
+                               3 << 21, 1 << 23, 3 << 22])
 def test_fused_lengths(brp, oracle, wl, n):
     w = wl.plain_full(seed=5, n=n, threshold_scale=0.3)
     p = w.templates[0]
