@@ -110,6 +110,29 @@ def test_plain_mixed_radix_small(brp, oracle, wl):
     say(rep)
 
 
+@pytest.mark.parametrize("variant", ["selected", "full_complex_radix2"])
+@pytest.mark.parametrize("n", [64, 96, 128, 192, 1 << 10, 3 << 10, 1 << 12])
+def test_plain_smallest_lengths(brp, oracle, wl, n, variant):
+    """The reference accepts n in {2^k, 3 * 2^k}, 64 <= n (src/science.cpp:46-65); below
+    the fused engine's shapes the plain engine serves those lengths at the same bar."""
+    w = wl.plain_full(seed=11, n=n, threshold_scale=0.3)
+    p = w.templates[0]
+    say(f"[gpu] plain N={n} variant={variant}")
+    with brp.GPU(p, w.x, 256, 0, 1, snapshot=True, **PLAIN_VARIANTS[variant]) as g:
+        g.set_template(p)
+        g.run()
+        rep = parity.check_template(g.snapshot(0), oracle, w.x, p, f"plain/N={n}/{variant}")
+    say(rep)
+
+
+@pytest.mark.parametrize("n", [64, 3 << 12, 1 << 12])
+def test_fused_rejects_lengths_below_its_shapes(brp, wl, n):
+    w = wl.plain_full(seed=11, n=n, threshold_scale=0.3)
+    say(f"[gpu] fused N={n} (rejected)")
+    with pytest.raises(RuntimeError, match="unsupported series length"):
+        brp.GPU(w.templates[0], w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED)
+
+
 @pytest.mark.parametrize("engine", ["fused", "plain"])
 def test_full_size_2p22(brp, oracle, wl, engine):
     w = wl.plain_full(seed=42)
