@@ -134,6 +134,29 @@ def test_fused_rejects_lengths_below_its_shapes(brp, wl, n):
 
 
 @pytest.mark.parametrize("engine", ["fused", "plain"])
+@pytest.mark.parametrize("n,unpadded,tau", [(1 << 16, 49152, 0.05), (3 << 15, 60000, 0.05), (1 << 16, 65536, 0.0),
+                                            (1 << 17, 100000, 0.0)])
+def test_padding_and_zero_delay(brp, oracle, wl, engine, n, unpadded, tau):
+    """An observation shorter than the transform (a long mean pad, which the fused engine
+    adds by linearity in pass 2) and an orbit of zero delay (every index j = i)."""
+    w = wl.plain_full(seed=13, n=n, threshold_scale=0.3)
+    x = np.ascontiguousarray(w.x[:unpadded])
+    p = brp.parameters(n, unpadded, tau, wl.INJECTED[0], wl.INJECTED[2], wl.DT)
+    for h in range(5):
+        p.threshold[h] *= 0.3
+    say(f"[gpu] {engine} N={n} unpadded={unpadded} tau={tau} steps={p.steps}")
+    kw = dict(packed=True, engine=brp.ENGINE_FUSED) if engine == "fused" else PLAIN_VARIANTS["selected"]
+    with brp.GPU(p, x, 256, 0, 1, snapshot=True, **kw) as g:
+        g.set_template(p)
+        g.run()
+        snap = g.snapshot(0)
+        rep = parity.check_template(snap, oracle, x, p, f"{engine}/N={n}/U={unpadded}/tau={tau}")
+    if tau == 0.0:
+        assert np.array_equal(snap.index[:p.steps], np.arange(p.steps))
+    say(rep)
+
+
+@pytest.mark.parametrize("engine", ["fused", "plain"])
 def test_full_size_2p22(brp, oracle, wl, engine):
     w = wl.plain_full(seed=42)
     p = w.templates[0]
