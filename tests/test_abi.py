@@ -107,3 +107,32 @@ int main() {
     assert int(steps) == p.steps and int(fundamental) == p.fundamental
     assert float(s0) == pytest.approx(p.s0, rel=1e-7) and float(lut) == pytest.approx(brp.lut_sin(1.25), rel=1e-7)
     assert out[1].startswith("error: invalid observation/template")
+
+
+def test_entry_points_reject_null_arguments(brp):
+    """Every handle entry point checks its pointers before any device call: a null handle or
+    output pointer is an error status with a message, never a crash."""
+    import ctypes
+
+    lib = brp.load()
+    p = brp.parameters(1 << 12, 1 << 12, 0.01, 600.0, 0.5, 64e-6)
+    n = ctypes.c_size_t()
+    calls = [
+        ("brp_hip_create", lambda: lib.brp_hip_create(None, None, 0, None, None)),
+        ("brp_hip_set_templates", lambda: lib.brp_hip_set_templates(None, ctypes.byref(p), 1)),
+        ("brp_hip_run", lambda: lib.brp_hip_run(None, None, None)),
+        ("brp_hip_candidates", lambda: lib.brp_hip_candidates(None, 0, None, 0, ctypes.byref(n))),
+        ("brp_hip_snapshot", lambda: lib.brp_hip_snapshot(None, 0, None, None, None, None, None)),
+        ("brp_hip_add_series", lambda: lib.brp_hip_add_series(None, ctypes.byref(p), None, 0)),
+        ("brp_hip_bank_search",
+         lambda: lib.brp_hip_bank_search(None, None, 0, 0, 0.0, 0.0, 0.0, 0, None, None, None, None)),
+        ("brp_hip_bank_search_templates",
+         lambda: lib.brp_hip_bank_search_templates(None, None, None, 0, None, 0, None, None)),
+        ("brp_hip_bank_search_device",
+         lambda: lib.brp_hip_bank_search_device(None, None, None, 0, None, 0, None, None)),
+    ]
+    for name, call in calls:
+        rc = call()
+        assert rc != 0, name
+        assert lib.brp_last_error(), name
+    assert lib.brp_hip_destroy(None) == 0  # like delete of a null pointer
