@@ -6,6 +6,8 @@ the oracle at full size (its FP64 FFT takes about a second there).
 """
 import sys
 
+import math
+
 import numpy as np
 import pytest
 
@@ -273,6 +275,40 @@ def test_fused_batch_slots_match_single_runs(brp, wl):
             g.set_template(p)
             g.run()
             assert np.array_equal(g.candidates(0), batched[s])
+
+
+@pytest.mark.parametrize("flags", [0, 6])
+def test_fused_random_orbits_index_exact(brp, oracle, wl, flags):
+    """48 seeded orbits in one batch (P log-uniform in [660, 10000] s, tau in [0, 0.3] s):
+    the first 24 with psi in [0, 2 pi) (launch groups that take the non-negative index
+    form), the rest with psi in (-2 pi, 2 pi) (the general form).  Every slot's resample
+    indices equal the oracle's bit for bit, and its power, sums and candidates meet the bar."""
+    n, count = 1 << 16, 48
+    rng = np.random.default_rng(2026 + flags)
+    w = wl.plain_small(threshold_scale=0.5)
+    templates = []
+    for k in range(count):
+        period = float(np.exp(rng.uniform(np.log(660.0), np.log(10000.0))))
+        tau = float(rng.uniform(0.0, 0.3))
+        psi = float(rng.uniform(0.0, 2 * np.pi) if k < count // 2 else rng.uniform(-2 * np.pi, 2 * np.pi))
+        p = brp.parameters(n, n, tau, period, psi, wl.DT)
+        if flags:
+            p.flags = flags
+            p.s0 = float(np.float32(np.float32(p.tau) * np.float32(math.sin(np.float32(p.psi)))) * np.float32(p.step_inv))
+            brp.resampling_steps(p)
+        for h in range(5):
+            p.threshold[h] *= 0.5
+        templates.append(p)
+    say(f"[gpu] fused N=2^16, {count} random orbits in one batch, flags={flags}")
+    with brp.GPU(templates[0], w.x, 256, 0, count, packed=True, engine=brp.ENGINE_FUSED, snapshot=True,
+                 group=8) as g:
+        g.set_templates(templates)
+        g.run()
+        cands = 0
+        for s, p in enumerate(templates):
+            rep = parity.check_template(g.snapshot(s), oracle, w.x, p, f"fused/random orbit {s}/flags={flags}")
+            cands += rep["candidates"]["gpu"]
+    say({"templates": count, "candidates": cands})
 
 
 @pytest.mark.parametrize("graph", [True, False])
