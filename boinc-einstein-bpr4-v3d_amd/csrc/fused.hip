@@ -386,12 +386,15 @@ k_pass2(FusedArgs a, FusedShape sh) {
     const uint32_t slot = a.first + g;
     const brp_params& p = a.params[slot];
     const uint32_t C = sh.c, M = sh.m, N = sh.n;
-    const bool middle = w == C / (2 * S);  // the block that owns the self-paired column C/2
-    const uint32_t base = middle ? C / 2 : w * S;
-
+    const uint32_t base = w * S;
+    // Sequence s < S is the lower column base + s, sequence S + s its mirror C - base - s.
+    // Columns 0 and C/2 are their own mirrors: block 0 keeps column 0 in sequence 0 and
+    // takes column C/2 in sequence S, so C/(2S) blocks cover every column once.
+    const bool first_block = w == 0;
     auto column = [&](int s) -> uint32_t {
         const uint32_t lower = base + uint32_t(s % S);
-        return s < S ? lower : (C - lower) % C;
+        if (s < S) return lower;
+        return (first_block && s == S) ? C / 2 : C - lower;
     };
 
     // This block's share of the pass-1 partial sums, read before the columns so that the
@@ -468,23 +471,11 @@ k_pass2(FusedArgs a, FusedShape sh) {
         const int pi = t + T * u;
         Wk[u] = twiddle2(a.twn_lo, a.twn_hi, uint32_t(pi / S) * C + base + uint32_t(pi % S));
     }
-#pragma unroll
-    for (int u = 0; u < PAIRS; ++u) {
-        const int pi = t + T * u;
-        const int s = pi % S;
-        const uint32_t k1 = uint32_t(pi / S);
-        if (u) we = mul(we, wstep);
-        if (middle && s != 0) continue;
-        const uint32_t col = base + uint32_t(s);
-        const bool self = col == 0 || col == C / 2;
-        const uint32_t k1m = col ? (R - 1 - k1) : (R - k1) % R;
-        const uint32_t k = k1 * C + col;
-        const cf zk = load(lds + lds_at<SS>(int(k1), s));
-        const cf zm = load(lds + lds_at<SS>(int(k1m), s + S));
-        const cf W = Wk[u];
+    // X[k] and X[M - k] from Z[k] and Z[M - k] (zk, zm), W = W_N^k, we = W_N^{k steps}.
+    auto real_split = [&](const cf zk, const cf zm, const cf W, const cf we_k, uint32_t k, cf& xk, cf& xm) {
         const cf Wm = cf{-W.x, W.y};  // W_N^{M-k} = -conj(W_N^k)
-        cf xk = scale(add(add(zk, conj(zm)), mul_negi(mul(sub(zk, conj(zm)), W))), 0.5f);
-        cf xm = scale(add(add(zm, conj(zk)), mul_negi(mul(sub(zm, conj(zk)), Wm))), 0.5f);
+        xk = scale(add(add(zk, conj(zm)), mul_negi(mul(sub(zk, conj(zm)), W))), 0.5f);
+        xm = scale(add(add(zm, conj(zk)), mul_negi(mul(sub(zm, conj(zk)), Wm))), 0.5f);
         if (k == 0) {
             xk.x += mu * pad_len;
             xm.x += mu * (parity - 1.0f) * 0.5f;  // U[M] = ((-1)^steps - 1) / 2
@@ -496,16 +487,48 @@ k_pass2(FusedArgs a, FusedShape sh) {
             const float den = front ? -W.y : 1.0f - W.x;
             const float ck = num * __builtin_amdgcn_rcpf(den);
             const float cm = den * __builtin_amdgcn_rcpf(num);
-            const cf uk = mul(cf{we.x - 1.0f, we.y}, cf{0.5f, -0.5f * ck});
-            const cf um = mul(cf{parity * we.x - 1.0f, -parity * we.y}, cf{0.5f, -0.5f * cm});
+            const cf uk = mul(cf{we_k.x - 1.0f, we_k.y}, cf{0.5f, -0.5f * ck});
+            const cf um = mul(cf{parity * we_k.x - 1.0f, -parity * we_k.y}, cf{0.5f, -0.5f * cm});
             xk = cf{__builtin_fmaf(mu, uk.x, xk.x), __builtin_fmaf(mu, uk.y, xk.y)};
             xm = cf{__builtin_fmaf(mu, um.x, xm.x), __builtin_fmaf(mu, um.y, xm.y)};
         }
+    };
+#pragma unroll
+    for (int u = 0; u < PAIRS; ++u) {
+        const int pi = t + T * u;
+        const int s = pi % S;
+        const uint32_t k1 = uint32_t(pi / S);
+        if (u) we = mul(we, wstep);
+        const uint32_t col = base + uint32_t(s);
+        const bool self = col == 0;
+        // Column 0 pairs k1 with R - k1 within its own sequence.
+        const uint32_t k1m = self ? (R - k1) % R : R - 1 - k1;
+        const int ms = self ? 0 : s + S;
+        const uint32_t k = k1 * C + col;
+        const cf zk = load(lds + lds_at<SS>(int(k1), s));
+        const cf zm = load(lds + lds_at<SS>(int(k1m), ms));
+        cf xk, xm;
+        real_split(zk, zm, Wk[u], we, k, xk, xm);
         P[k] = k == 0 ? 0.0f : (xk.x * xk.x + xk.y * xk.y) * inv_n;
         if (!self) P[M - k] = (xm.x * xm.x + xm.y * xm.y) * inv_n;
         if constexpr (SNAP) {
             store(snapX + k, xk);
             if (!self || k == 0) store(snapX + (M - k), xm);
+        }
+    }
+    // Column C/2 (sequence S of block 0) pairs k1 with R - 1 - k1 within itself; each
+    // thread takes rows t, t + T, ... and writes X[k] only.
+    if (first_block) {
+        for (uint32_t k1 = uint32_t(t); k1 < R; k1 += T) {
+            const uint32_t k = k1 * C + C / 2;
+            const cf zk = load(lds + lds_at<SS>(int(k1), S));
+            const cf zm = load(lds + lds_at<SS>(int(R - 1 - k1), S));
+            const cf W = twiddle2(a.twn_lo, a.twn_hi, k);
+            const cf we_k = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(k) * steps_n));
+            cf xk, xm;
+            real_split(zk, zm, W, we_k, k, xk, xm);
+            P[k] = (xk.x * xk.x + xk.y * xk.y) * inv_n;
+            if constexpr (SNAP) store(snapX + k, xk);
         }
     }
 }
@@ -784,7 +807,7 @@ bool shape_of(const ShapeRow& row, uint32_t n, FusedShape* out) {
     FusedShape s;
     s.n = n; s.m = n / 2; s.c = row.c; s.r = row.r; s.s1 = row.s1; s.s2 = row.s2; s.t1 = row.t1; s.t2 = row.t2;
     s.blocks1 = row.r / row.s1;
-    s.blocks2 = row.c / (2 * row.s2) + 1;
+    s.blocks2 = row.c / (2 * row.s2);
     s.mirror = row.s2 <= 4;
     *out = s;
     return true;
