@@ -640,6 +640,44 @@ def test_candidate_overflow_raises(brp, wl):
                 g.candidates(0)
 
 
+def test_device_retain_reports_invalid_candidates(brp):
+    """The device retain checks every candidate as the host does: a level past 4, a bin
+    outside [W, F) or a non-finite power is an error, and so is a level with more
+    candidates in one template than the retain can sort."""
+    n = 1 << 20
+    orbit = (3600.0, 0.05, 1.0)
+    p = brp.search_parameters(n, n, 64.0, orbit)
+    bad = {
+        "level 5": [brp.Candidate(p.window + 10, 5, 100.0, 0)],
+        "bin below W": [brp.Candidate(p.window - 1, 0, 100.0, 0)],
+        "bin at F": [brp.Candidate(p.fundamental, 1, 100.0, 0)],
+        "NaN power": [brp.Candidate(p.window + 3, 2, float("nan"), 0)],
+    }
+    for name, cands in bad.items():
+        say(f"[gpu] device retain rejects {name}")
+        with pytest.raises(RuntimeError, match="invalid compact candidate"):
+            brp.toplist_retain_device(brp.Toplist(100), [p], [orbit], [cands])
+    many = [brp.Candidate(p.window + i, 0, 100.0 + i, 0) for i in range(5000)]
+    say("[gpu] device retain rejects 5000 candidates of one level")
+    with pytest.raises(RuntimeError, match="more than 4096"):
+        brp.toplist_retain_device(brp.Toplist(100), [p], [orbit], [many])
+
+
+def test_bank_search_device_reports_overflow(brp, wl):
+    """A template whose candidates overflow its capacity makes the device-toplist bank
+    search fail, as the host loop does, instead of retaining a partial list."""
+    w = wl.bank(count=16, n=1 << 16)
+    templates = [t.copy() for t in w.templates]
+    for t in templates:
+        t.capacity = 4
+        for h in range(5):
+            t.threshold[h] *= 0.2
+    say("[gpu] device-toplist bank search with capacity 4 and low thresholds")
+    with brp.GPU(templates[0], w.x, 256, 0, 16, packed=True, engine=brp.ENGINE_FUSED, graph=True) as g:
+        with pytest.raises(RuntimeError, match="overflow"):
+            g.bank_search_device(templates, w.orbits, [brp.Toplist(100)])
+
+
 def test_out_of_range_index_flagged(brp, wl):
     w = wl.plain_small()
     # psi = 2.5: the orbital offset falls by ~4.6 samples over the series, so the last
