@@ -136,3 +136,26 @@ def test_entry_points_reject_null_arguments(brp):
         assert rc != 0, name
         assert lib.brp_last_error(), name
     assert lib.brp_hip_destroy(None) == 0  # like delete of a null pointer
+
+
+def test_preprocess_validates_before_touching_the_device(brp):
+    """brp_hip_preprocess checks the DD metadata, the lengths and the window before any
+    device call, with the reference's messages (src/workload.cpp:37-45, 57)."""
+    import numpy as np
+
+    block = np.zeros(512, np.uint8)
+    cases = [
+        (dict(samples=1024, bits=3, n=1024), "unsupported DD sample format"),
+        (dict(samples=1023, bits=4, n=1024), "odd packed DD length"),
+        (dict(samples=1000, bits=4, n=1000), "observation exceeds supported length"),
+        (dict(samples=2048, bits=4, n=1024), "invalid DD metadata"),
+        (dict(samples=1024, bits=4, n=1024, window=63), "unsupported whitening window"),
+        (dict(samples=1024, bits=4, n=1024, window=1024), "unsupported whitening window"),
+        (dict(samples=1024, bits=4, n=1024, rfi_ratio=-1.0), "invalid RFI ratio"),
+        (dict(samples=1024, bits=4, n=1024, zap=[(1.0, 2.0)], sample_us=64.0, padding=0.5), "invalid zap arguments"),
+    ]
+    for kw, message in cases:
+        args = dict(window=64, rfi_ratio=0.0)
+        args.update(kw)
+        with pytest.raises(RuntimeError, match=message):
+            brp.preprocess(block, args.pop("samples"), args.pop("bits"), 1.0, args.pop("n"), **args)
