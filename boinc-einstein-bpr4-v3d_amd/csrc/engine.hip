@@ -207,6 +207,7 @@ struct brp_hip {
     }
     ~brp_hip() {
         if (stream) hipStreamSynchronize(stream);
+        if (side) hipStreamSynchronize(side);
         if (top_stream) hipStreamSynchronize(top_stream);
         for (auto g : graph_exec) if (g) hipGraphExecDestroy(g);
         for (auto& list : events) for (hipEvent_t e : list) hipEventDestroy(e);

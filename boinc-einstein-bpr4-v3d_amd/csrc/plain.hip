@@ -7,6 +7,9 @@
 // pad mean, global Stockham radix-2/4/8 passes with twiddles from one table of
 // W_fft_n^t computed in double.  The fused engine (fused.hip) is the fast path; this one
 // is the baseline "one template per launch" GPU path and an independent cross-check.
+#include <stdexcept>
+#include <string>
+
 #include "device.h"
 #include "engine.h"
 
@@ -259,7 +262,8 @@ void plain_spectrum(hipStream_t st, const PlainBuffers& b, const float2* z, bool
     if (b.packed) {
         k_real<<<blocks(b.n / 2 + 1), kBlock, 0, st>>>(z, b.tw_real, b.n / 2, inv_n, b.X, fused_power ? power : nullptr);
     } else {
-        hipMemcpyAsync(b.X, z, sizeof(float2) * (b.n / 2 + 1), hipMemcpyDeviceToDevice, st);
+        const hipError_t e = hipMemcpyAsync(b.X, z, sizeof(float2) * (b.n / 2 + 1), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) throw std::runtime_error(std::string("copy spectrum: ") + hipGetErrorString(e));
     }
 }
 
