@@ -600,6 +600,7 @@ void fill_timings(brp_hip& h, int half, uint32_t count) {
 // Enqueues the templates of `half` (parameters up, kernels, packed results down) and
 // records its completion event; nothing waits.
 void launch(brp_hip& h, int half) {
+    (void)hipGetLastError();  // so that the check after the launches sees only theirs
     const uint32_t count = uint32_t(h.templates[half].size());
     const size_t base = h.slot_base(half);
     for (uint32_t i = 0; i < count; ++i) {
@@ -633,6 +634,8 @@ void launch(brp_hip& h, int half) {
         }
         run_fused(h, half);
     }
+    // A launch the runtime rejected (configuration, resources) leaves its error here only.
+    HIPC(hipGetLastError());
     if (h.device_top) {
         // Retains run in batch order on top_stream, each after its batch's kernels.
         HIPC(hipEventRecord(h.kernels_done[half], h.stream));
@@ -640,6 +643,7 @@ void launch(brp_hip& h, int half) {
         brp_engine::toplist_retain(h.top_stream, h.results + base * h.slot_words, h.slot_words, h.params_dev + base,
                                    count, h.slot_table_dev + base, h.slot_orbit_dev + base, h.dtop_tables, h.dtop,
                                    h.dtop_rows, h.dtop_lowest[half], h.dtop_error);
+        HIPC(hipGetLastError());
         HIPC(hipEventRecord(h.retained[half], h.top_stream));
         h.retain_pending[half] = true;
     }
