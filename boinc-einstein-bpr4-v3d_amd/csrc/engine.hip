@@ -958,11 +958,13 @@ int brp_hip_autotune(const brp_params* templates, size_t count, const float* inp
 int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps) {
     return guarded([&] {
         require(gbps && bytes >= 16 && reps > 0, "invalid copy calibration");
+        (void)hipGetLastError();
         bytes &= ~size_t(15);
         void *a = nullptr, *b = nullptr;
         hipStream_t st = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         auto release = [&] {
+            if (st) hipStreamSynchronize(st);
             if (e0) hipEventDestroy(e0);
             if (e1) hipEventDestroy(e1);
             if (st) hipStreamDestroy(st);
@@ -979,6 +981,7 @@ int brp_hip_copy_bandwidth(size_t bytes, int reps, double* gbps) {
             brp_engine::copy16(st, a, b, bytes);  // warm-up
             HIPC(hipEventRecord(e0, st));
             for (int r = 0; r < reps; ++r) brp_engine::copy16(st, a, b, bytes);
+            HIPC(hipGetLastError());
             HIPC(hipEventRecord(e1, st));
             HIPC(hipEventSynchronize(e1));
             float ms = 0;
