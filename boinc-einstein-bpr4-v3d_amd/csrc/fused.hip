@@ -720,7 +720,7 @@ bool with_pass1(uint32_t c, uint32_t s1, uint32_t t1, F&& f) {
     switch (inst(c, s1, t1)) {
 #define BRP_P1(C, S, T) case inst(C, S, T): f.template operator()<C, S, T>(); return true;
         BRP_P1(64, 64, 256) BRP_P1(128, 32, 256) BRP_P1(256, 16, 256) BRP_P1(512, 16, 512)
-        BRP_P1(1024, 16, 1024) BRP_P1(2048, 8, 1024) BRP_P1(2048, 4, 512) BRP_P1(2048, 2, 256)
+        BRP_P1(1024, 16, 1024) BRP_P1(2048, 8, 1024) BRP_P1(2048, 4, 512) BRP_P1(2048, 2, 256) BRP_P1(2048, 2, 512)
         BRP_P1(4096, 4, 1024) BRP_P1(4096, 2, 512) BRP_P1(4096, 1, 256)
         BRP_P1(4096, 2, 1024) BRP_P1(4096, 1, 512)
         BRP_P1(384, 32, 512) BRP_P1(768, 16, 512) BRP_P1(1536, 8, 512) BRP_P1(3072, 2, 256)
@@ -790,6 +790,7 @@ constexpr ShapeRow kShapes[] = {
     // eight elements per thread (radix-8 plans, fewer registers, more waves per CU)
     {1u << 21, 4096, 512, 2, 4, 1024, 256},    {1u << 21, 4096, 512, 1, 4, 512, 256},
     {1u << 21, 4096, 512, 2, 4, 512, 512},     {1u << 21, 4096, 512, 1, 4, 512, 512},
+    {1u << 21, 2048, 1024, 2, 4, 512, 512},
 };
 
 }  // namespace

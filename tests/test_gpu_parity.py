@@ -199,6 +199,8 @@ ALTERNATIVE_SPLITS = [
     (1 << 22, 4096, 1, 4, 512, 512),
     # 4096-point rows in 512-thread blocks, 256-thread pass-2 blocks
     (1 << 22, 4096, 2, 4, 512, 256),
+    # 2048-point rows, eight elements per thread (radix 8, 8, 8, 4)
+    (1 << 22, 2048, 2, 4, 512, 512),
     # the earlier defaults of 2^20, 2^21, 3 * 2^19 and 3 * 2^20
     (1 << 20, 2048, 8, 16, 1024, 512), (1 << 21, 4096, 4, 16, 1024, 512), (3 << 19, 2048, 4, 8, 512, 256),
     (3 << 20, 4096, 2, 8, 512, 256)]
