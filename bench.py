@@ -363,6 +363,9 @@ def main() -> int:
     if not args.no_cpu:
         import oracle_lib
 
+        # One template per thread on the host's cores: every core of the affinity set, up
+        # to 16 (the host share that comes with one GPU on a shared node; the affinity set
+        # can name every core of the node).  Both counts are reported.
         cores = len(os.sched_getaffinity(0))
         threads = min(cores, 16)
         sample = w.templates[:threads]
@@ -374,9 +377,11 @@ def main() -> int:
             ms = oracle_lib.benchmark_cpu(w.x, sample, threads, min(reps, 5))
         ms1 = oracle_lib.benchmark_cpu(w.x, w.templates[:1], 1, 3)
         cpu = {"value": round(len(sample) / (ms * 1e-3), 3), "unit": "templates/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{len(sample)} bank templates at N={n} (one per thread), resample + FP32 Stockham FFT "
-                         f"(not FFTW) + power + harmonic + select, median of repetitions after one warm-up",
+               "affinity_cores": cores, "kind": "port",
+               "sample": f"{len(sample)} bank templates at N={n} (one per thread on {threads} threads), the "
+                         f"oracle's C++ port of the reference CPU path: resample + its own FP32 Stockham FFT "
+                         f"(the reference's FFTW3f is not in the image) + power + harmonic + select, median of "
+                         f"repetitions after one warm-up",
                "single_core": {"value": round(1.0 / (ms1 * 1e-3), 3), "unit": "templates/s", "cores": 1,
                                "sample": "bank template 0, median of 3 after one warm-up"}}
         log(f"[bench] cpu baseline {cpu}")

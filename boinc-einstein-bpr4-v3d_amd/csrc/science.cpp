@@ -111,14 +111,24 @@ float brp_sample_offset(uint32_t i, const brp_params* p) {
 // The largest steps <= unpadded - 1 with float(steps) - offset(steps) < unpadded - 1 in
 // the FP32 rule (src/science.cpp:41-45).  The reference scans down from unpadded - 1,
 // up to ~2 tau/dt iterations.  Here the scan starts 64 samples above the FP64 solution
-// s* of s - offset(s) = unpadded - 1: s - offset(s) grows with slope 1 - tau omega cos >
-// 0.5, and the FP32 rule differs from the FP64 model by well under one sample, so no
-// s beyond that start can satisfy the rule and the result is the reference's.
+// s* of s - offset(s) = unpadded - 1 when that is provably safe: s - offset(s) grows with
+// slope 1 - tau omega cos > 0.5, so an FP32-vs-FP64 offset difference of e samples moves
+// the solution by at most 2e, and no s more than 2e above s* can satisfy the rule.  The
+// offset difference is at most (tau/dt) x E with E the FP32 LUT sine's error against
+// sin: the Taylor remainder |d|^3/6 <= 1.98e-5 at |d| <= pi/64, the table's 6-digit
+// rounding 5e-7 x (1 + |d| + d^2/2), the reduction by 6.283185 instead of 2 pi (3.1e-7 per
+// whole cycle of the phase) and the FP32 roundings of the phase (a few ulp of |phase|):
+// E <= 2.5e-5 + 1e-6 (1 + |phase|).  With (tau/dt) E above 15 samples (tau/dt beyond about
+// 5e5 samples) the reference's own scan from unpadded - 1 runs instead.
 int brp_resampling_steps(brp_params* p) {
     if (!p || p->unpadded < 2) return brp::fail(BRP_E_INVALID, "invalid template");
     const float last = float(p->unpadded - 1);
     uint32_t steps = p->unpadded - 1;
-    if (double(p->tau) * double(p->omega) < 0.5) {
+    const double delay = std::fabs(double(p->tau) * double(p->step_inv));  // tau/dt in samples
+    const double max_phase = std::fabs(double(p->omega)) * double(p->unpadded) * std::fabs(double(p->dt)) +
+                             std::fabs(double(p->psi));
+    const double offset_error = delay * (2.5e-5 + 1e-6 * (1.0 + max_phase));
+    if (double(p->tau) * double(p->omega) < 0.5 && offset_error <= 15.0) {
         auto offset = [&](double s) {
             return double(p->tau) * std::sin(double(p->omega) * (s * double(p->dt)) + double(p->psi)) *
                        double(p->step_inv) - double(p->s0);

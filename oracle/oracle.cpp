@@ -581,6 +581,43 @@ int oracle_preprocess_zap(const uint8_t* bytes, size_t samples, uint32_t bits, d
     return 0;
 }
 
+int oracle_bank_candidates(const float* input, size_t count, const oracle_params* tpl, size_t ntpl,
+                           uint32_t threads, uint32_t fft32, oracle_candidate* out, size_t cap, uint32_t* counts) {
+    if (!threads) threads = 1;
+    std::vector<std::string> errors(threads);
+    std::vector<std::thread> pool;
+    for (uint32_t w = 0; w < threads; ++w)
+        pool.emplace_back([&, w] {
+            std::vector<float> rs, spec, pw, sum;
+            std::vector<oracle_candidate> tmp(cap + 1);
+            for (size_t t = w; t < ntpl; t += threads) {
+                const oracle_params& p = tpl[t];
+                rs.resize(p.n); spec.resize(p.n + 2); pw.resize(p.n / 2); sum.resize(5 * size_t(p.fundamental));
+                size_t k;
+                if (fft32) {
+                    cpu_template(input, count, p, rs, spec, pw, sum, tmp);
+                    k = oracle_select(sum.data(), &p, out + t * cap, cap);
+                } else {
+                    if (oracle_reference(input, count, &p, rs.data(), pw.data(), sum.data())) {
+                        errors[w] = g_error;
+                        return;
+                    }
+                    k = oracle_select(sum.data(), &p, out + t * cap, cap);
+                }
+                if (k > cap) {
+                    errors[w] = "template " + std::to_string(t) + " has " + std::to_string(k) + " candidates (cap " +
+                                std::to_string(cap) + ")";
+                    return;
+                }
+                counts[t] = uint32_t(k);
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (const auto& e : errors)
+        if (!e.empty()) return fail(e);
+    return 0;
+}
+
 double oracle_benchmark_cpu(const float* input, size_t count, const oracle_params* tpl,
                             size_t ntpl, uint32_t threads, uint32_t reps) {
     if (!threads) threads = 1;

@@ -90,6 +90,19 @@ int oracle_toplist_retain(oracle_bank_row* state, uint32_t rows, const oracle_ca
 size_t oracle_toplist_final(const oracle_bank_row* state, uint32_t rows, oracle_bank_row* out,
                             size_t cap);
 
+/* The bank loop's per-template work (src/workunit_main.cpp:77-99) over `ntemplates`
+ * templates on `threads` threads: resample -> FP64 power (rounded to float) -> harmonic ->
+ * select, each template with its own thresholds as given (the bank's base thresholds).
+ * fft32 != 0 takes the FP32 FFT and power of the CPU baseline instead of the FP64 ones.
+ * Candidates of template t go to out[t * cap .. t * cap + counts[t]).  A raised threshold
+ * only removes candidates (a level's value is its best sum when that passes the threshold),
+ * so replaying oracle_toplist_thresholds + oracle_toplist_retain over these lists in
+ * template order is the reference's one-template-at-a-time loop.  Fails if a template has
+ * more than cap candidates. */
+int oracle_bank_candidates(const float* input, size_t count, const oracle_params* templates,
+                           size_t ntemplates, uint32_t threads, uint32_t fft32, oracle_candidate* out,
+                           size_t cap, uint32_t* counts);
+
 /* ---- CPU baseline (methodology of src/reference_adapter.cpp:30-56) ---- */
 /* Runs resample -> FP32 FFT -> power -> harmonic -> select for `templates` templates on
  * `threads` threads (one template per thread at a time), `reps` timed repetitions after one

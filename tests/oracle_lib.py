@@ -6,6 +6,7 @@ cpu_baseline leg.  The product (libbrp_hip.so and brp_hip.py) never imports it.
 from __future__ import annotations
 
 import ctypes
+import os
 import pathlib
 import subprocess
 
@@ -87,6 +88,7 @@ def load() -> ctypes.CDLL:
         "oracle_toplist_retain": (i32, [P(ORow), u32, P(OCandidate), sz, f64, f64, f64, P(OParams)]),
         "oracle_toplist_final": (sz, [P(ORow), u32, P(ORow), sz]),
         "oracle_benchmark_cpu": (f64, [fp, sz, P(OParams), sz, u32, u32]),
+        "oracle_bank_candidates": (i32, [fp, sz, P(OParams), sz, u32, u32, P(OCandidate), sz, P(u32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -222,6 +224,39 @@ class OToplist:
         out = (ORow * self.rows)()
         n = load().oracle_toplist_final(self.state, self.rows, out, self.rows)
         return list(out[:n])
+
+
+def worker_threads() -> int:
+    """Host threads for the oracle's thread pool: the affinity set, at most 16 (the host
+    share of one GPU)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def bank_candidates(x: np.ndarray, templates, threads: int = 0, cap: int = 8192, fft32: bool = False) -> list:
+    """Per-template candidate lists (CAND_DTYPE, (level, bin) order) of the bank loop's
+    per-template pipeline with each template's own thresholds, on a thread pool; fft32: the
+    CPU baseline's FP32 FFT and power instead of the FP64 ones."""
+    arr = (OParams * len(templates))(*[as_oparams(t) for t in templates])
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(len(templates) * cap, dtype=CAND_DTYPE)
+    counts = np.zeros(len(templates), dtype=np.uint32)
+    _check(load().oracle_bank_candidates(_fp(x), x.size, arr, len(templates), threads or worker_threads(), int(fft32),
+                                         out.ctypes.data_as(ctypes.POINTER(OCandidate)), cap,
+                                         counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+    return [out[t * cap:t * cap + int(counts[t])].copy() for t in range(len(templates))]
+
+
+def bank_toplists(templates, orbits, candidates, rows_list=(100,)) -> list:
+    """src/workunit_main.cpp:77-136 one template at a time: for each toplist size, raise each
+    template's thresholds from the toplist as it stands (src/workload.cpp:126) and retain its
+    candidates (src/workload.cpp:129-141), in template order."""
+    tops = [OToplist(rows) for rows in rows_list]
+    for p, o, c in zip(templates, orbits, candidates):
+        for top in tops:
+            q = as_oparams(p)
+            top.thresholds(q)
+            top.retain(c, o, q)
+    return tops
 
 
 def benchmark_cpu(x: np.ndarray, templates, threads: int, reps: int) -> float:

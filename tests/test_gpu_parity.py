@@ -572,9 +572,10 @@ def test_uneven_workunit_pipeline(brp, oracle, wl):
 
 @pytest.mark.parametrize("n", [3 << 20, 1 << 22])
 def test_split_fp16_mfma_variant(brp, oracle, wl, n):
-    """precision = 1: the pass-2 radix-16 stages as split-FP16 MFMA GEMMs.  Indices and
-    the resampled series are unchanged; the power meets the 1e-5 parity bar against the FP64
-    oracle, and its max / RMS error is reported beside the FP32 path's."""
+    """precision = 1: the radix-16 stages as split-FP16 MFMA GEMMs.  Indices and the
+    resampled series are unchanged; the power is within 2e-6 relative of the FP64 oracle
+    (the variant's target; the parity bar is 1e-5), and its max / RMS error is reported
+    beside the FP32 path's."""
     w = wl.mixed_radix(count=2, n=n) if n % 3 == 0 else wl.plain_full(seed=9, n=n, threshold_scale=0.3)
     p = w.templates[0]
     errs = {}
@@ -589,7 +590,7 @@ def test_split_fp16_mfma_variant(brp, oracle, wl, n):
         if prec:
             assert " R=1024 " in info or " R=2048 " in info  # a shape with radix-16 pass-2 stages
     say({"fp32": errs[0], "split_fp16_mfma": errs[1]})
-    assert errs[1]["max_rel"] <= 1e-5
+    assert errs[1]["max_rel"] <= 2e-6  # the variant's target against the FP64 oracle
 
 
 def test_split_fp16_rejected_without_radix16_stages(brp, wl):

@@ -59,6 +59,29 @@ def test_resampling_steps_equal_oracle_on_extreme_orbits(brp, oracle):
             assert _bytes(a) == _bytes(b)
 
 
+def test_resampling_steps_long_delays_equal_the_linear_scan(brp, oracle):
+    """Delays of 10^4 .. 2 x 10^7 samples (tau up to 1280 s at dt = 64 us, tau omega < 0.5):
+    past the jump start's error bound the reference's scan from unpadded - 1 runs, and every
+    result equals the oracle's linear scan (src/science.cpp:41-45)."""
+    rng = np.random.default_rng(7)
+    n = 1 << 22
+    for tau in (0.64, 6.4, 32.0, 64.0, 160.0, 320.0):
+        for _ in range(3):
+            period = float(rng.uniform(2 * np.pi * tau / 0.45, 4 * np.pi * tau / 0.45))
+            psi = float(rng.uniform(0, 6.28))
+            a = brp.parameters(n, n - int(rng.integers(0, 1000)), tau, period, psi, 64e-6)
+            b = oracle.parameters(n, a.unpadded, tau, period, psi, 64e-6)
+            assert _bytes(a) == _bytes(b), (tau, period, psi)
+    # Orbits whose FP32 rule lies 30-140 samples beyond the FP64 solution: a scan started
+    # 64 samples above that solution would stop short of the reference's length.
+    for tau, period, psi in ((640.0, 10219.499304313382, 2.7828623112678965),
+                             (640.0, 14349.413763432298, 3.783723419434997),
+                             (1280.0, 28940.997487526118, 2.8567255598987424)):
+        a = brp.parameters(n, n, tau, period, psi, 64e-6)
+        b = oracle.parameters(n, n, tau, period, psi, 64e-6)
+        assert _bytes(a) == _bytes(b), (tau, period, psi)
+
+
 def test_parameter_validation_messages(brp):
     with pytest.raises(RuntimeError, match="invalid observation/template"):
         brp.parameters(100, 100, 0.01, 100.0, 0.1, 64e-6)
