@@ -28,7 +28,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
              f"-I{ROOT / 'include'}", f"-I{CSRC}"]
-HIP_SOURCES = ["plain.hip", "fused.hip", "preprocess.hip", "toplist.hip", "engine.hip"]
+HIP_SOURCES = ["plain.hip", "fused.hip", "preprocess.hip", "toplist.hip", "engine.hip", "noalloc.hip"]
 CLI_SOURCES = ["brp_bench.cpp", "brp_workunit.cpp"]
 BIN = PKG / "bin"
 CXX_SOURCES = ["science.cpp"]

@@ -133,6 +133,7 @@ def load() -> ctypes.CDLL:
         "brp_hip_autotune": (i32, [P(Params), sz, fp, sz, P(Options), u32, P(Options), ctypes.c_char_p, sz]),
         "brp_hip_bank_search": (i32, [ctypes.c_void_p, P(Orbit), sz, u32, f64, f64, f64, u32, ctypes.c_void_p,
                                       P(Params), P(u32), P(ctypes.c_double)]),
+        "brp_hip_search_noalloc": (i32, [P(Params), sz, ctypes.c_void_p, sz, ctypes.c_void_p, ctypes.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -245,6 +246,30 @@ def autotune(templates, x: np.ndarray, batch_options: "dict | None" = None, reps
     report = [json.loads(line) for line in buf.value.decode().splitlines() if line.strip()]
     return {"split_c": best.split_c, "split_s1": best.split_s1, "split_s2": best.split_s2,
             "split_t1": best.split_t1, "split_t2": best.split_t2, "group": best.group, "lanes": best.lanes}, report
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20
+NOALLOC_MAX_TEMPLATES = 8
+
+
+def search_noalloc(templates, input_ptr: int, input_count: int, results_ptr: int, stream: int = 0) -> None:
+    """brp_hip_search_noalloc: the fused pipeline for up to 8 templates of one series with
+    its buffers in the library's static device scratch; input_ptr / results_ptr are device
+    addresses (input_count floats readable, count slots of 16 + 16 * capacity bytes), stream
+    a hipStream_t handle (0: default stream).  Only enqueues: synchronize before reading."""
+    arr = (Params * len(templates))(*templates)
+    _check(load().brp_hip_search_noalloc(arr, len(arr), ctypes.c_void_p(input_ptr), input_count,
+                                         ctypes.c_void_p(results_ptr), ctypes.c_void_p(stream)))
+
+
+def slot_candidates(words: np.ndarray, capacity: int, slot: int) -> tuple[np.ndarray, int, int]:
+    """(candidates sorted by (level, bin), count, flags) of result slot `slot` in a uint32
+    array of slots of 4 + 4 * capacity words ({count, flags, 0, 0}, then candidates)."""
+    rec = words[slot * (4 + 4 * capacity):(slot + 1) * (4 + 4 * capacity)]
+    count, flags = int(rec[0]), int(rec[1])
+    c = rec[4:4 + 4 * min(count, capacity)].view(CANDIDATE_DTYPE).copy()
+    c.sort(order=["level", "bin"])
+    return c, count, flags
 
 
 def copy_bandwidth(nbytes: int, reps: int = 10) -> float:

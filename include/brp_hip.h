@@ -287,6 +287,28 @@ int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint3
                         brp_toplist* top, brp_params* templates_out, uint32_t* cand_count,
                         double* gpu_ms);
 
+/* --------------------------------------------------- no allocation -------- */
+/* No counterpart in the reference.  For callers that cannot allocate device memory: code
+ * under HIP graph capture, serving loops and harnesses that forbid hipMalloc.  Runs the fused
+ * engine's pipeline (resample + pass 1, pass 2 + power, harmonic sums + candidates) for
+ * `count` templates (1 .. BRP_NOALLOC_MAX_TEMPLATES) of one series, with every intermediate
+ * buffer (parameters, LUT, twiddle tables, exchange, power, partial sums) in a file-scope
+ * device array of BRP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  The
+ * templates must share n, unpadded, capacity and the arithmetic flags; N x count must fit the
+ * array (N = 2^22: 2 templates; N = 2^20: 8), else BRP_E_INVALID with the sizes.
+ *   input:   device pointer to the series (input_count >= unpadded + 2 readable floats; the
+ *            two after the series are read and never used);
+ *   results: device pointer to count result slots of 16 + 16 * capacity bytes each, filled as
+ *            the handle's slots are ({count, flags, 0, 0}, then candidates in no order; flag
+ *            1 = candidate overflow, 2 = a resample index outside the series);
+ *   stream:  a hipStream_t (NULL: the default stream).  The call only enqueues work.
+ * Nothing is allocated and nothing is copied from host memory, so the call can be captured.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU race. */
+#define BRP_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define BRP_NOALLOC_MAX_TEMPLATES 8
+int brp_hip_search_noalloc(const brp_params* templates, size_t count, const float* input, size_t input_count,
+                           uint32_t* results, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
