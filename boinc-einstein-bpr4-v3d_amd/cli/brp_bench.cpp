@@ -1,245 +1,208 @@
-// brp_bench -- the reference's single-template benchmark driver (src/main.cpp:12-118) on
-// the HIP backend: one template over a synthetic series, checked, then replayed `reps`
-// times; prints one JSON object with the reference's fields.
+// brp_bench -- single-template benchmark driver over the HIP backend, with the reference's
+// command line, JSON report and exit codes (src/main.cpp:12-118): one template over a
+// synthetic series is checked, then replayed `--reps` times in batches of `--batch`.
 //
-// Options are the reference's (--n --unpadded --wg --tile --stockham --radix4 --radix8
-// --packed --scatter --production --fused-power --radix8-store --counters --batch --reps
-// --seed --tau --period --psi --dt --threshold-scale --allow-software), plus:
-//   --engine fused|plain   fused: the batched four-step engine (default); plain: the
-//                          reference's launch sequence
-//   --group G              fused engine: templates per launch (default 8)
+// Options: the reference's --n --unpadded --wg --tile --batch --reps --seed --tau --period
+// --psi --dt --threshold-scale and switches --stockham --radix4 --radix8 --radix8-store
+// --packed --scatter --production --fused-power --counters --allow-software, plus
+//   --engine fused|plain   the batched four-step engine (default) or the reference's
+//                          launch sequence
+//   --group G              fused engine: templates per launch
 // Differences, by necessity:
-//   * the input is N(0, 1) noise from this library's counter-based generator plus the
-//     reference's 37 Hz cosines (0.5, 0.25, 0.15 at 1x, 2x, 3x; src/science.cpp:66-74):
-//     the reference's std::normal_distribution is implementation-defined;
-//   * `valid` checks the engine against the other engine on the device (the CPU path of
-//     this build is test infrastructure): bit-equal resample indices, FFT / power /
-//     harmonic rel-L2, candidates and top-100 ranks per level, replay reproducibility;
+//   * the series is this library's seeded N(0, 1) noise plus the reference's 37 Hz cosines
+//     (amplitudes 0.5, 0.25, 0.15 at 1x, 2x, 3x; src/science.cpp:66-74); the reference's
+//     std::normal_distribution draws are implementation-defined;
+//   * `valid` checks this engine against the other one on the device, since this build's CPU
+//     path is test infrastructure: bit-equal resampled samples, spectrum / power / harmonic
+//     rel-L2, candidates, top-100 ranks per level, and bit-equal replays;
 //     harmonic_isolated_l2 and the CPU timing fields are null.
-// Exit status 0 when valid, 2 when not, 1 on an error (src/main.cpp:117-118).
-#include <sys/resource.h>
-
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <iomanip>
-#include <iostream>
+// Exit status 0 when valid, 2 when not, 1 on an error.
 #include <numbers>
 
+#include "driver_kit.hpp"
 #include "workload_io.hpp"
 
 namespace {
 
-double median(std::vector<double> x) {
-    std::sort(x.begin(), x.end());
-    return x[x.size() / 2];
+struct HelpRequested {};
+
+struct Settings {
+    uint32_t n = 65536, unpadded = 0, wg = 192, tile = 512, batch = 4, reps = 9, seed = 42, group = 0;
+    float tau = 0.002f, period = 120, psi = 0.3f, dt = 0.000256f, threshold_scale = 1;
+    bool software = false, stockham = false, radix4 = false, radix8 = false, radix8_store = false, packed = false,
+         scatter = false, production = false, fused_power = false, counters = false;
+    std::string engine = "fused";
+};
+
+driver::Options command_line(Settings& s) {
+    driver::Options o({"option needs value", "unknown option: ", std::nullopt});
+    using driver::to;
+    o.value("--n", to(s.n), "--n 2^k|3*2^k")
+        .value("--unpadded", to(s.unpadded), "--unpadded N")
+        .value("--wg", to(s.wg), "--wg 16..256")
+        .value("--tile", to(s.tile), "--tile 0|64..4096")
+        .flag("--stockham", [&] { s.stockham = true; }, "--stockham --tile 0")
+        .flag("--radix4", [&] { s.radix4 = true; }, "--radix4|--radix8")
+        .flag("--radix8", [&] { s.radix8 = true; })
+        .flag("--packed", [&] { s.packed = true; }, "--packed")
+        .flag("--scatter", [&] { s.scatter = true; }, "--scatter")
+        .flag("--production", [&] { s.production = true; }, "--production")
+        .flag("--fused-power", [&] { s.fused_power = true; }, "--fused-power")
+        .flag("--counters", [&] { s.counters = true; }, "--counters --batch 1")
+        .value("--batch", to(s.batch), "--batch 1..64")
+        .value("--reps", to(s.reps), "--reps N")
+        .value("--seed", to(s.seed), "--seed N")
+        .value("--tau", to(s.tau), "--tau seconds")
+        .value("--period", to(s.period), "--period seconds")
+        .value("--psi", to(s.psi), "--psi radians")
+        .value("--dt", to(s.dt), "--dt seconds")
+        .value("--threshold-scale", to(s.threshold_scale), "--threshold-scale factor")
+        .flag("--allow-software", [&] { s.software = true; }, "--allow-software")
+        .value("--engine", to(s.engine), "--engine fused|plain")
+        .value("--group", to(s.group), "--group G")
+        .flag("--radix8-store", [&] { s.radix8 = s.radix8_store = true; })
+        .flag("--help", [] { throw HelpRequested{}; });
+    return o;
 }
 
-void require(bool b, const char* s) {
-    if (!b) throw std::runtime_error(s);
-}
-
-const char* tf(bool b) { return b ? "true" : "false"; }
-
-std::span<const float> floats(const std::vector<std::complex<float>>& x) {
-    return {reinterpret_cast<const float*>(x.data()), 2 * x.size()};
-}
-
-std::vector<float> synthetic(uint32_t n, float dt, uint32_t seed) {
-    std::vector<float> x(n);
-    brp::check(brp_synthetic_series(n, dt, seed, 0.0, 1.0, 0.0, 0.0, 0.0, nullptr, x.data()));
-    for (uint32_t i = 0; i < n; ++i) {
-        const double phase = 2 * std::numbers::pi * 37.0 * i * dt;
-        x[i] += float(0.5 * std::cos(phase) + 0.25 * std::cos(2 * phase) + 0.15 * std::cos(3 * phase));
+// The reference's bench series shape: noise plus 37 Hz and its 2nd and 3rd harmonics.
+std::vector<float> bench_series(uint32_t count, float dt, uint32_t seed) {
+    std::vector<float> x(count);
+    brp::hip_detail::check(brp_synthetic_series(count, dt, seed, 0.0, 1.0, 0.0, 0.0, 0.0, nullptr, x.data()));
+    constexpr double amplitude[3] = {0.5, 0.25, 0.15};
+    for (uint32_t i = 0; i < count; ++i) {
+        const double w = 2 * std::numbers::pi * 37.0 * double(i) * double(dt);
+        double tone = 0;
+        for (int k = 0; k < 3; ++k) tone += amplitude[k] * std::cos(double(k + 1) * w);
+        x[i] += float(tone);
     }
     return x;
 }
 
-}  // namespace
-
-int main(int argc, char** argv) try {
-    uint32_t n = 65536, unpadded = 0, wg = 192, tile = 512, batch = 4, reps = 9, seed = 42, group = 0;
-    float tau = 0.002f, period = 120, psi = 0.3f, dt = 0.000256f, threshold_scale = 1;
-    bool software = false, stockham = false, radix4 = false, packed = false, scatter = false, production = false,
-         radix8 = false, counters = false, fused_power = false, radix8_store = false;
-    std::string engine_name = "fused";
-    for (int i = 1; i < argc; ++i) {
-        const std::string key = argv[i];
-        if (key == "--help") {
-            std::cout << "brp_bench [--n 2^k|3*2^k] [--unpadded N] [--wg 16..256] [--tile 0|64..4096] [--stockham --tile 0] "
-                         "[--radix4|--radix8] [--packed] [--scatter] [--production] [--fused-power] [--counters --batch 1] "
-                         "[--batch 1..64] [--reps N] [--seed N] [--tau seconds] [--period seconds] [--psi radians] "
-                         "[--dt seconds] [--threshold-scale factor] [--allow-software] [--engine fused|plain] "
-                         "[--group G]\n";
-            return 0;
-        }
-        if (key == "--allow-software") { software = true; continue; }
-        if (key == "--stockham") { stockham = true; continue; }
-        if (key == "--radix4") { radix4 = true; continue; }
-        if (key == "--packed") { packed = true; continue; }
-        if (key == "--scatter") { scatter = true; continue; }
-        if (key == "--production") { production = true; continue; }
-        if (key == "--radix8") { radix8 = true; continue; }
-        if (key == "--counters") { counters = true; continue; }
-        if (key == "--fused-power") { fused_power = true; continue; }
-        if (key == "--radix8-store") { radix8_store = true; radix8 = true; continue; }
-        require(i + 1 < argc, "option needs value");
-        const std::string v = argv[++i];
-        if (key == "--n") n = std::stoul(v);
-        else if (key == "--unpadded") unpadded = std::stoul(v);
-        else if (key == "--wg") wg = std::stoul(v);
-        else if (key == "--tile") tile = std::stoul(v);
-        else if (key == "--batch") batch = std::stoul(v);
-        else if (key == "--reps") reps = std::stoul(v);
-        else if (key == "--seed") seed = std::stoul(v);
-        else if (key == "--tau") tau = std::stof(v);
-        else if (key == "--period") period = std::stof(v);
-        else if (key == "--psi") psi = std::stof(v);
-        else if (key == "--dt") dt = std::stof(v);
-        else if (key == "--threshold-scale") threshold_scale = std::stof(v);
-        else if (key == "--engine") engine_name = v;
-        else if (key == "--group") group = std::stoul(v);
-        else throw std::runtime_error("unknown option: " + key);
+int bench(int argc, char** argv) {
+    Settings s;
+    driver::Options cl = command_line(s);
+    try {
+        cl.parse(argc, argv);
+    } catch (const HelpRequested&) {
+        std::cout << cl.usage("brp_bench") << '\n';
+        return 0;
     }
-    require(reps > 0 && reps <= 10000 && std::isfinite(threshold_scale) && threshold_scale >= 0,
-            "invalid repetitions/thresholds");
-    require(engine_name == "fused" || engine_name == "plain", "--engine must be fused or plain");
-    const uint32_t engine = engine_name == "fused" ? BRP_ENGINE_FUSED : BRP_ENGINE_PLAIN;
-    if (!unpadded) unpadded = n - 17;
-    brp::Params p = brp::parameters(n, unpadded, tau, period, psi, dt);
-    if (production) {  // src/main.cpp:40
+    if (!(s.reps > 0 && s.reps <= 10000 && std::isfinite(s.threshold_scale) && s.threshold_scale >= 0))
+        throw std::runtime_error("invalid repetitions/thresholds");
+    if (s.engine != "fused" && s.engine != "plain") throw std::runtime_error("--engine must be fused or plain");
+    const uint32_t engine = s.engine == "fused" ? BRP_ENGINE_FUSED : BRP_ENGINE_PLAIN;
+    if (!s.unpadded) s.unpadded = s.n - 17;
+
+    brp::Params p = brp::parameters(s.n, s.unpadded, s.tau, s.period, s.psi, s.dt);
+    if (s.production) {  // flag-2 arithmetic with its own s0 and length (src/main.cpp:40)
         p.flags = 2 | 4;
         p.s0 = (p.tau * std::sin(p.psi)) * p.step_inv;
         brp::resampling_steps(p);
     }
-    for (auto& t : p.threshold) t *= threshold_scale;
-    const std::vector<float> input = synthetic(unpadded, dt, seed);
+    for (float& t : p.threshold) t *= s.threshold_scale;
+    const std::vector<float> series = bench_series(s.unpadded, s.dt, s.seed);
 
     brp_options opt{};
-    opt.workgroup = wg; opt.tile = tile; opt.batch = batch; opt.software = software; opt.stockham = stockham;
-    opt.radix4 = radix4; opt.packed = packed; opt.scatter = scatter; opt.radix8 = radix8; opt.fused_power = fused_power;
-    opt.radix8_store = radix8_store; opt.engine = engine; opt.group = group;
+    opt.workgroup = s.wg;
+    opt.tile = s.tile;
+    opt.batch = s.batch;
+    opt.software = s.software;
+    opt.stockham = s.stockham;
+    opt.radix4 = s.radix4;
+    opt.radix8 = s.radix8;
+    opt.radix8_store = s.radix8_store;
+    opt.packed = s.packed;
+    opt.scatter = s.scatter;
+    opt.fused_power = s.fused_power;
+    opt.engine = engine;
+    opt.group = s.group;
     opt.graph = engine == BRP_ENGINE_FUSED;
 
-    // Checks: this engine and the other one, one template each, with snapshots.
-    auto checker = [&](uint32_t which) {
+    // The check: one template on this engine and on the other, both with snapshots.
+    auto single = [&](uint32_t which) {
         brp_options o = opt;
         o.engine = which;
         o.batch = 1;
         o.snapshot = 1;
         o.graph = 0;
-        if (which != engine) { o.tile = 0; o.stockham = 1; }
-        auto g = std::make_unique<brp::GPU>(p, input, o);
+        if (which != engine) {
+            o.tile = 0;
+            o.stockham = 1;
+        }
+        auto g = std::make_unique<brp::GPU>(p, series, o);
         g->set_template(p);
         g->run();
         return g;
     };
-    auto mine = checker(engine);
-    auto other = checker(engine == BRP_ENGINE_FUSED ? BRP_ENGINE_PLAIN : BRP_ENGINE_FUSED);
-    const brp::Snapshot actual = mine->snapshot(), ref = other->snapshot();
-    uint32_t index_mismatches = 0;
-    for (uint32_t i = 0; i < p.steps; ++i) index_mismatches += actual.resampled[i] != ref.resampled[i];
-    const double resamp_error = brp::relative_l2(actual.resampled, ref.resampled);
-    const double fft_error = brp::relative_l2(floats(actual.fft), floats(ref.fft));
-    const double power_error = brp::relative_l2(actual.power, ref.power);
-    const double hs_error = brp::relative_l2(actual.summed, ref.summed);
-    uint32_t candidate_mismatches = 0;
-    if (actual.candidates.size() != ref.candidates.size()) ++candidate_mismatches;
-    else
-        for (size_t i = 0; i < ref.candidates.size(); ++i) {
-            const auto a = actual.candidates[i], b = ref.candidates[i];
-            if (a.bin != b.bin || a.level != b.level || std::abs(a.power - b.power) > 2e-4f * (1 + std::abs(b.power)))
-                ++candidate_mismatches;
-        }
-    uint32_t rank_mismatches = 0;
-    for (uint32_t h = 0; h < 5; ++h) {
-        std::vector<brp::Candidate> a, b;
-        for (auto c : actual.candidates) if (c.level == h) a.push_back(c);
-        for (auto c : ref.candidates) if (c.level == h) b.push_back(c);
-        auto order = [](auto x, auto y) { return x.power != y.power ? x.power > y.power : x.bin < y.bin; };
-        std::sort(a.begin(), a.end(), order);
-        std::sort(b.begin(), b.end(), order);
-        for (size_t i = 0; i < std::min<size_t>(100, std::min(a.size(), b.size())); ++i)
-            rank_mismatches += a[i].bin != b[i].bin;
-    }
-    bool valid = !index_mismatches && !candidate_mismatches && !rank_mismatches && resamp_error < 5e-5 &&
-                 fft_error < 3e-5 && power_error < 5e-5 && hs_error < 5e-5;
+    const auto checked = single(engine);
+    const auto other = single(engine == BRP_ENGINE_FUSED ? BRP_ENGINE_PLAIN : BRP_ENGINE_FUSED);
+    const brp::Snapshot first = checked->snapshot();
+    const driver::Check check = driver::compare(first, other->snapshot(), p.steps);
 
-    // The timed handle: `batch` copies of the template, replayed `reps` times.
-    const auto init_start = std::chrono::steady_clock::now();
-    brp::GPU gpu(p, input, opt);
-    const double init_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - init_start).count();
-    gpu.set_templates(std::vector<brp::Params>(batch, p));
+    // The timed handle: `batch` copies of the template; two untimed runs, then `reps`.
+    const auto t_init = std::chrono::steady_clock::now();
+    brp::GPU gpu(p, series, opt);
+    const double init_ms = driver::ms_since(t_init);
+    gpu.set_templates(std::vector<brp::Params>(s.batch, p));
     gpu.run();
-    gpu.run();  // untimed warm-up (graph capture)
-    std::vector<double> wall, cpu, gpu_ms, submit_ms;
-    std::array<std::vector<double>, 4> stages;
-    double total_wall = 0, total_cpu = 0;
-    for (uint32_t r = 0; r < reps; ++r) {
+    gpu.run();
+    driver::Series wall, cpu, gpu_ms, submit;
+    std::array<driver::Series, 4> stage;
+    double wall_total = 0, cpu_total = 0;
+    for (uint32_t r = 0; r < s.reps; ++r) {
         const brp::Timings t = gpu.run();
-        wall.push_back(t.wall_ms / batch);
-        cpu.push_back(t.cpu_ms / batch);
-        total_wall += t.wall_ms;
-        total_cpu += t.cpu_ms;
-        submit_ms.push_back(t.submit_ms / batch);
-        double sum = 0;
-        for (uint32_t s = 0; s < 4; ++s) {
-            stages[s].push_back(t.stage_ms[s] / batch);
-            sum += t.stage_ms[s];
+        const double per = 1.0 / s.batch;
+        wall.add(t.wall_ms * per);
+        cpu.add(t.cpu_ms * per);
+        submit.add(t.submit_ms * per);
+        wall_total += t.wall_ms;
+        cpu_total += t.cpu_ms;
+        double device = 0;
+        for (size_t k = 0; k < 4; ++k) {
+            stage[k].add(t.stage_ms[k] * per);
+            device += t.stage_ms[k];
         }
-        gpu_ms.push_back(sum / batch);
+        gpu_ms.add(device * per);
     }
-    // Replay reproducibility: the checked handle again, bit for bit.
-    mine->run();
-    const brp::Snapshot final = mine->snapshot();
-    bool reproducible = actual.resampled == final.resampled && actual.fft == final.fft && actual.power == final.power &&
-                        actual.summed == final.summed && actual.candidates.size() == final.candidates.size();
-    if (reproducible)
-        for (size_t i = 0; i < actual.candidates.size(); ++i) {
-            const auto a = actual.candidates[i], b = final.candidates[i];
-            if (a.bin != b.bin || a.level != b.level || a.power != b.power) reproducible = false;
-        }
-    valid = valid && reproducible;
-    const std::string counter_json = counters ? gpu.profile_counters() : "null";
-    rusage usage{};
-    getrusage(RUSAGE_SELF, &usage);
-    std::cout << std::setprecision(10) << "{\n  \"device\": \"" << gpu.device_info() << "\",\n"
-              << "  \"engine\": \"" << engine_name << "\", \"group\": " << group << ",\n"
-              << "  \"n\": " << n << ", \"unpadded\": " << unpadded << ", \"steps\": " << p.steps << ", \"seed\": " << seed
-              << ",\n"
-              << "  \"tau\": " << tau << ", \"period\": " << period << ", \"psi\": " << psi << ", \"dt\": " << dt
-              << ", \"threshold_scale\": " << threshold_scale << ",\n"
-              << "  \"wg\": " << wg << ", \"tile\": " << tile << ", \"batch\": " << batch << ", \"reps\": " << reps << ",\n"
-              << "  \"stockham\": " << tf(stockham) << ", \"packed\": " << tf(packed) << ", \"scatter\": " << tf(scatter)
-              << ", \"production\": " << tf(production) << ",\n"
-              << "  \"fused_power\": " << tf(fused_power) << ",\n"
-              << "  \"radix8_store\": " << tf(radix8_store) << ",\n"
-              << "  \"twiddle_layout\": \"dense\",\n"
-              << "  \"radix4\": " << tf(radix4) << ", \"radix8\": " << tf(radix8) << ", \"submission_ms_median\": "
-              << median(submit_ms) << ",\n"
-              << "  \"valid\": " << tf(valid) << ", \"index_mismatches\": " << index_mismatches
-              << ", \"candidate_mismatches\": " << candidate_mismatches << ", \"rank_mismatches\": " << rank_mismatches
-              << ",\n"
-              << "  \"replay_reproducible\": " << tf(reproducible) << ",\n"
-              << "  \"resample_l2\": " << resamp_error << ", \"fft_l2\": " << fft_error << ", \"power_l2\": "
-              << power_error << ", \"harmonic_l2\": " << hs_error << ", \"harmonic_isolated_l2\": null,\n"
-              << "  \"candidates\": " << actual.candidates.size() << ", \"initialization_ms\": " << init_ms
-              << ", \"cpu_reference_with_planning_ms\": null,\n"
-              << "  \"cpu_core_wall_ms_median\": null, \"cpu_core_process_ms_median\": null,\n"
-              << "  \"wall_ms_median\": " << median(wall) << ", \"wall_ms_min\": "
-              << *std::min_element(wall.begin(), wall.end()) << ", \"wall_ms_max\": "
-              << *std::max_element(wall.begin(), wall.end()) << ",\n"
-              << "  \"gpu_ms_median\": " << median(gpu_ms) << ", \"process_cpu_ms_median\": " << median(cpu)
-              << ", \"cpu_percent\": " << 100 * total_cpu / total_wall << ",\n"
-              << "  \"replays_per_second\": " << 1000 / median(wall) << ", \"allocated_gpu_bytes\": "
-              << gpu.allocated_bytes() << ", \"peak_rss_kib\": " << usage.ru_maxrss << ",\n"
-              << "  \"stage_ms_median\": {\"resample\": " << median(stages[0]) << ", \"fft\": " << median(stages[1])
-              << ", \"power\": " << median(stages[2]) << ", \"harmonic_candidate\": " << median(stages[3])
-              << "},\n  \"wall_ms_samples\": [";
-    for (size_t i = 0; i < wall.size(); ++i) std::cout << (i ? ", " : "") << wall[i];
-    std::cout << "],\n  \"hardware_counters\": " << counter_json << "\n}\n";
+    checked->run();
+    const bool reproducible = driver::identical(first, checked->snapshot());
+    const bool valid = check.within_bars(true) && reproducible;
+
+    driver::Json stages;
+    stages.put("resample", stage[0].median()).put("fft", stage[1].median()).put("power", stage[2].median());
+    stages.put("harmonic_candidate", stage[3].median());
+    driver::Json out;
+    out.put("device", gpu.device_info()).put("engine", s.engine).put("group", s.group);
+    out.put("n", s.n).put("unpadded", s.unpadded).put("steps", p.steps).put("seed", s.seed);
+    out.put("tau", s.tau).put("period", s.period).put("psi", s.psi).put("dt", s.dt);
+    out.put("threshold_scale", s.threshold_scale);
+    out.put("wg", s.wg).put("tile", s.tile).put("batch", s.batch).put("reps", s.reps);
+    out.put("stockham", s.stockham).put("packed", s.packed).put("scatter", s.scatter).put("production", s.production);
+    out.put("fused_power", s.fused_power).put("radix8_store", s.radix8_store);
+    out.put("twiddle_layout", BRP_STRIDED_TWIDDLES ? "strided" : "dense");
+    out.put("radix4", s.radix4).put("radix8", s.radix8).put("submission_ms_median", submit.median());
+    out.put("valid", valid).put("index_mismatches", check.index_mismatches);
+    out.put("candidate_mismatches", check.candidate_mismatches).put("rank_mismatches", check.rank_mismatches);
+    out.put("replay_reproducible", reproducible);
+    out.put("resample_l2", check.resample_l2).put("fft_l2", check.fft_l2).put("power_l2", check.power_l2);
+    out.put("harmonic_l2", check.harmonic_l2).put("harmonic_isolated_l2", nullptr);
+    out.put("candidates", first.candidates.size()).put("initialization_ms", init_ms);
+    out.put("cpu_reference_with_planning_ms", nullptr);
+    out.put("cpu_core_wall_ms_median", nullptr).put("cpu_core_process_ms_median", nullptr);
+    out.put("wall_ms_median", wall.median()).put("wall_ms_min", wall.min()).put("wall_ms_max", wall.max());
+    out.put("gpu_ms_median", gpu_ms.median()).put("process_cpu_ms_median", cpu.median());
+    out.put("cpu_percent", 100 * cpu_total / wall_total);
+    out.put("replays_per_second", 1000 / wall.median()).put("allocated_gpu_bytes", gpu.allocated_bytes());
+    out.put("peak_rss_kib", driver::usage().peak_rss_kib);
+    out.put("stage_ms_median", stages).list("wall_ms_samples", wall.v);
+    out.raw("hardware_counters", s.counters ? gpu.profile_counters() : "null");
+    std::cout << out.text(true) << '\n';
     return valid ? 0 : 2;
-} catch (const std::exception& e) {
-    std::cerr << "brp_bench: " << e.what() << '\n';
-    return 1;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    return driver::guarded("brp_bench", [&] { return bench(argc, argv); });
 }

@@ -2,8 +2,8 @@
 // src/workunit_main.cpp:15-25): the DD observation (gzip or plain, a 1168-byte header of
 // little-endian doubles and the sample count, then 4-bit or 8-bit samples), the orbital
 // bank (one "period tau psi" row per line), the zap list ("lower upper" Hz pairs) and raw
-// float32 files.  Host code of the command-line drivers; the decode itself runs on the
-// device (brp_hip_preprocess_zap).
+// float32 files.  Host code of the command-line drivers over brp/hip_science.hpp's
+// Observation; the decode and preprocessing run on the device (brp_hip_preprocess_zap).
 #pragma once
 #include <zlib.h>
 
@@ -21,16 +21,9 @@
 #include <vector>
 
 #include "brp/hip.hpp"
+#include "brp/hip_science.hpp"
 
 namespace brp {
-
-// include/brp/workload.hpp:6-10, with the undecoded sample block kept for the device.
-struct Observation {
-    double sample_us{}, observation_s{}, dm{}, scale{}, padding{};
-    uint32_t samples{}, n{}, bits{};
-    std::vector<uint8_t> block;
-    std::vector<float> decoded, preprocessed;
-};
 
 // src/workload.cpp:29-56: header fields sample_us (double 0), observation_s (double 1),
 // dm (double 13), scale (double 14), samples (uint32 at byte 128); n = padding *
@@ -139,25 +132,6 @@ inline void dump(const std::filesystem::path& path, std::span<const float> value
     std::ofstream file(path, std::ios::binary);
     file.write(reinterpret_cast<const char*>(values.data()), std::streamsize(values.size_bytes()));
     if (!file) throw std::runtime_error("state dump failed");
-}
-
-// src/workload.cpp:160-165
-inline void write_candidates(const std::filesystem::path& path, std::span<const BankCandidate> rows, double seconds) {
-    check(brp_write_candidates(path.c_str(), rows.data(), rows.size(), seconds));
-}
-
-// Relative L2 distance |a - b| / |b| in double, infinite when any value is not finite
-// (the metric of src/science.cpp:121-127).
-inline double relative_l2(std::span<const float> a, std::span<const float> b) {
-    if (a.size() != b.size()) throw std::runtime_error("comparison length mismatch");
-    double num = 0, den = 0;
-    for (size_t i = 0; i < a.size(); ++i) {
-        if (!std::isfinite(a[i]) || !std::isfinite(b[i])) return INFINITY;
-        const double d = double(a[i]) - double(b[i]);
-        num += d * d;
-        den += double(b[i]) * double(b[i]);
-    }
-    return std::sqrt(num / std::max(den, 1e-30));
 }
 
 }  // namespace brp

@@ -1,50 +1,68 @@
-// hip.hpp -- the reference's C++ backend interface (include/brp/vulkan.hpp:6-29 and the
-// science/workload declarations it uses) over the C ABI of brp_hip.h.  Header-only, so
-// callers written against the reference's brp::GPU compile unchanged after swapping
-// the include; errors surface as std::runtime_error with the library's message, as the
-// reference's backend throws.
+// hip.hpp -- drop-in for the reference's include/brp/vulkan.hpp: the same brp::Timings,
+// brp::Snapshot and brp::GPU (include/brp/vulkan.hpp:6-29) over the C ABI of brp_hip.h.
+//
+// A caller of the reference swaps `#include "brp/vulkan.hpp"` for `#include "brp/hip.hpp"`,
+// adds this include directory and links libbrp_hip.so in place of brp_vulkan.  Like
+// vulkan.hpp, this header includes "brp/science.hpp": inside the reference's tree that is
+// the reference's own science library (Params, Candidate, the CPU oracle), which the caller
+// keeps linking, and its workload library keeps providing Orbit, Toplist and
+// search_parameters.  Without the reference's headers it is this library's mirror,
+// brp/hip_science.hpp.  Params and Candidate cross the ABI as brp_params and brp_candidate,
+// whose layouts are checked member by member below.  Errors surface as std::runtime_error
+// with the library's message, as the reference's backend throws.
 #pragma once
+#if __has_include("brp/science.hpp")
+#include "brp/science.hpp"
+#else
+#include "brp/hip_science.hpp"
+#endif
+
 #include <array>
 #include <complex>
+#include <cstddef>
 #include <cstdint>
 #include <memory>
 #include <span>
-#include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
+#include "brp/hip_error.hpp"
 #include "brp_hip.h"
+
+// brp_vulkan exports this definition to its users (CMakeLists.txt:39); its twiddle tables
+// here are always dense.
+#ifndef BRP_STRIDED_TWIDDLES
+#define BRP_STRIDED_TWIDDLES 0
+#endif
 
 namespace brp {
 
-using Params = brp_params;        // include/brp/science.hpp:8-17
-using Candidate = brp_candidate;  // include/brp/science.hpp:18
-using Orbit = brp_orbit;          // include/brp/workload.hpp:13
-using BankCandidate = brp_bank_row;
-static_assert(sizeof(Params) == 108);
-static_assert(sizeof(Candidate) == 16);
+namespace hip_detail {
+// brp::Params and brp_params: the same 108-byte block (include/brp/science.hpp:8-17).
+static_assert(std::is_standard_layout_v<Params> && sizeof(Params) == sizeof(brp_params));
+#define BRP_HIP_SAME(T, C, f) static_assert(offsetof(T, f) == offsetof(C, f), #f)
+BRP_HIP_SAME(Params, brp_params, n); BRP_HIP_SAME(Params, brp_params, unpadded);
+BRP_HIP_SAME(Params, brp_params, steps); BRP_HIP_SAME(Params, brp_params, logn);
+BRP_HIP_SAME(Params, brp_params, tau); BRP_HIP_SAME(Params, brp_params, omega);
+BRP_HIP_SAME(Params, brp_params, psi); BRP_HIP_SAME(Params, brp_params, dt);
+BRP_HIP_SAME(Params, brp_params, step_inv); BRP_HIP_SAME(Params, brp_params, s0);
+BRP_HIP_SAME(Params, brp_params, fundamental); BRP_HIP_SAME(Params, brp_params, harmonic);
+BRP_HIP_SAME(Params, brp_params, window); BRP_HIP_SAME(Params, brp_params, stage);
+BRP_HIP_SAME(Params, brp_params, threshold); BRP_HIP_SAME(Params, brp_params, capacity);
+BRP_HIP_SAME(Params, brp_params, fft_n); BRP_HIP_SAME(Params, brp_params, fft_base);
+BRP_HIP_SAME(Params, brp_params, fft_radix); BRP_HIP_SAME(Params, brp_params, flags);
+BRP_HIP_SAME(Params, brp_params, twiddle3); BRP_HIP_SAME(Params, brp_params, twiddle_real);
+BRP_HIP_SAME(Params, brp_params, result_slot);
+// brp::Candidate and brp_candidate (include/brp/science.hpp:18).
+static_assert(std::is_standard_layout_v<Candidate> && sizeof(Candidate) == sizeof(brp_candidate));
+BRP_HIP_SAME(Candidate, brp_candidate, bin); BRP_HIP_SAME(Candidate, brp_candidate, level);
+BRP_HIP_SAME(Candidate, brp_candidate, power); BRP_HIP_SAME(Candidate, brp_candidate, reserved);
+#undef BRP_HIP_SAME
 
-inline void check(int rc) {
-    if (rc != BRP_OK) throw std::runtime_error(brp_last_error());
-}
-
-// src/science.cpp:46-65
-inline Params parameters(uint32_t n, uint32_t unpadded, float tau, float period, float psi, float dt) {
-    Params p{};
-    check(brp_parameters(n, unpadded, tau, period, psi, dt, &p));
-    return p;
-}
-inline void resampling_steps(Params& p) { check(brp_resampling_steps(&p)); }
-inline float lut_sin(float phase) { return brp_lut_sin(phase); }
-inline float sample_offset(uint32_t i, const Params& p) { return brp_sample_offset(i, &p); }
-
-// src/workload.cpp:108-125 with the observation's (n, samples, sample_us).
-inline Params search_parameters(uint32_t n, uint32_t samples, double sample_us, const Orbit& orbit,
-                                double f0 = 400, double false_alarm = 0.08, uint32_t window = 1000) {
-    Params p{};
-    check(brp_search_parameters(n, samples, sample_us, &orbit, f0, false_alarm, window, &p));
-    return p;
-}
+inline const brp_params* c_params(const Params* p) { return reinterpret_cast<const brp_params*>(p); }
+inline brp_candidate* c_candidates(Candidate* c) { return reinterpret_cast<brp_candidate*>(c); }
+}  // namespace hip_detail
 
 // include/brp/vulkan.hpp:6
 struct Timings {
@@ -60,155 +78,117 @@ struct Snapshot {
     std::vector<Candidate> candidates;
 };
 
-// include/brp/workload.hpp:16-23
-class Toplist {
-public:
-    explicit Toplist(uint32_t rows = 100) {
-        brp_toplist* t = nullptr;
-        check(brp_toplist_create(rows, &t));
-        impl_.reset(t);
-    }
-    void thresholds(Params& p) const { brp_toplist_thresholds(impl_.get(), &p); }
-    void retain(std::span<const Candidate> c, const Orbit& orbit, const Params& p) {
-        check(brp_toplist_retain(impl_.get(), c.data(), c.size(), &orbit, &p));
-    }
-    std::vector<BankCandidate> final() const {
-        std::vector<BankCandidate> out(100000);
-        out.resize(brp_toplist_final(impl_.get(), out.data(), out.size()));
-        return out;
-    }
-    brp_toplist* handle() { return impl_.get(); }
-
-private:
-    struct Free { void operator()(brp_toplist* t) const { brp_toplist_destroy(t); } };
-    std::unique_ptr<brp_toplist, Free> impl_;
-};
-
-// include/brp/vulkan.hpp:12-29
+// include/brp/vulkan.hpp:12-29.  The reference's constructor runs the plain engine (the
+// reference's launch sequence, one kernel per stage); the brp_options constructor reaches
+// every backend option, the batched fused engine among them.
 class GPU {
 public:
     GPU(const Params& p, std::span<const float> input, uint32_t workgroup, uint32_t tile, uint32_t batch,
         bool software = false, bool stockham = false, bool radix4 = false, bool packed = false, bool scatter = false,
-        bool radix8 = false, bool fused_power = false, bool radix8_store = false,
-        uint32_t engine = BRP_ENGINE_PLAIN, bool graph = false, bool snapshot = false)
-        : p_(p), batch_(batch) {
-        brp_options o{};
-        o.workgroup = workgroup; o.tile = tile; o.batch = batch; o.software = software; o.stockham = stockham;
-        o.radix4 = radix4; o.packed = packed; o.scatter = scatter; o.radix8 = radix8; o.fused_power = fused_power;
-        o.radix8_store = radix8_store; o.engine = engine; o.graph = graph; o.snapshot = snapshot;
+        bool radix8 = false, bool fused_power = false, bool radix8_store = false)
+        : GPU(p, input, reference_options(workgroup, tile, batch, software, stockham, radix4, packed, scatter, radix8,
+                                          fused_power, radix8_store)) {}
+    GPU(const Params& p, std::span<const float> input, const brp_options& options) {
         brp_hip* h = nullptr;
-        check(brp_hip_create(&p, input.data(), input.size(), &o, &h));
-        impl_.reset(h);
-        templates_.assign(1, p);
+        hip_detail::check(brp_hip_create(hip_detail::c_params(&p), input.data(), input.size(), &options, &h));
+        handle_.reset(h);
+        current_.assign(1, p);
     }
-    // Every backend option (brp_options: the reference's switches plus engine, group,
-    // split, lanes and the other extensions).
-    GPU(const Params& p, std::span<const float> input, const brp_options& o) : p_(p), batch_(o.batch) {
-        brp_hip* h = nullptr;
-        check(brp_hip_create(&p, input.data(), input.size(), &o, &h));
-        impl_.reset(h);
-        templates_.assign(1, p);
-    }
+    ~GPU() = default;
     GPU(const GPU&) = delete;
     GPU& operator=(const GPU&) = delete;
 
     Timings run() {
         brp_timings t{};
-        std::vector<double> per(4 * templates_.size());
-        check(brp_hip_run(impl_.get(), &t, per.data()));
+        std::vector<double> per(4 * current_.size());
+        hip_detail::check(brp_hip_run(handle_.get(), &t, per.data()));
         Timings out{t.wall_ms, t.cpu_ms, t.submit_ms, {t.stage_ms[0], t.stage_ms[1], t.stage_ms[2], t.stage_ms[3]}, {}};
-        out.template_stage_ms.resize(templates_.size());
-        for (size_t r = 0; r < templates_.size(); ++r)
-            for (int s = 0; s < 4; ++s) out.template_stage_ms[r][s] = per[4 * r + s];
+        out.template_stage_ms.resize(current_.size());
+        for (size_t slot = 0; slot < current_.size(); ++slot)
+            for (size_t s = 0; s < 4; ++s) out.template_stage_ms[slot][s] = per[4 * slot + s];
         return out;
     }
     void set_template(const Params& p) { set_templates(std::span<const Params>(&p, 1)); }
     void set_templates(std::span<const Params> ps) {
-        check(brp_hip_set_templates(impl_.get(), ps.data(), ps.size()));
-        templates_.assign(ps.begin(), ps.end());
+        hip_detail::check(brp_hip_set_templates(handle_.get(), hip_detail::c_params(ps.data()), ps.size()));
+        current_.assign(ps.begin(), ps.end());
     }
+    // The last template of the batch: its resampled series with the mean pad, spectrum,
+    // power, harmonic sums and candidates (src/vulkan.cpp:319-328).
     Snapshot snapshot() const {
-        const uint32_t slot = uint32_t(templates_.size() - 1);
-        const Params& p = templates_.back();
+        const uint32_t slot = uint32_t(current_.size() - 1);
+        const Params& p = current_.back();
         Snapshot s;
         s.resampled.resize(p.n);
-        s.power.resize(p.n / 2);
-        s.summed.resize(5ull * p.fundamental);
         s.fft.resize(p.n / 2 + 1);
-        check(brp_hip_snapshot(impl_.get(), slot, s.resampled.data(), nullptr,
-                               reinterpret_cast<float*>(s.fft.data()), s.power.data(), s.summed.data()));
+        s.power.resize(p.n / 2);
+        s.summed.resize(5 * size_t(p.fundamental));
+        hip_detail::check(brp_hip_snapshot(handle_.get(), slot, s.resampled.data(), nullptr,
+                                           reinterpret_cast<float*>(s.fft.data()), s.power.data(), s.summed.data()));
         s.candidates = candidates(slot);
         return s;
     }
     std::vector<Candidate> candidates(uint32_t slot = UINT32_MAX) const {
-        if (slot == UINT32_MAX) slot = uint32_t(templates_.size() - 1);
-        size_t n = 0;
-        check(brp_hip_candidates(impl_.get(), slot, nullptr, 0, &n));
-        std::vector<Candidate> out(n);
-        check(brp_hip_candidates(impl_.get(), slot, out.data(), out.size(), &n));
+        if (slot == UINT32_MAX) slot = uint32_t(current_.size() - 1);
+        size_t count = 0;
+        hip_detail::check(brp_hip_candidates(handle_.get(), slot, nullptr, 0, &count));
+        std::vector<Candidate> out(count);
+        hip_detail::check(brp_hip_candidates(handle_.get(), slot, hip_detail::c_candidates(out.data()), out.size(),
+                                             &count));
         return out;
     }
-    std::string device_info() const {
-        std::string s(brp_hip_device_info(impl_.get(), nullptr, 0) + 1, '\0');
-        brp_hip_device_info(impl_.get(), s.data(), s.size());
-        s.pop_back();
-        return s;
-    }
-    uint64_t allocated_bytes() const { return brp_hip_allocated_bytes(impl_.get()); }
-    std::string profile_counters() {
-        std::string s(brp_hip_profile_counters(impl_.get(), nullptr, 0) + 1, '\0');
-        brp_hip_profile_counters(impl_.get(), s.data(), s.size());
-        s.pop_back();
-        return s;
-    }
-    brp_hip* handle() { return impl_.get(); }
+    std::string device_info() const { return text(brp_hip_device_info); }
+    uint64_t allocated_bytes() const { return brp_hip_allocated_bytes(handle_.get()); }
+    std::string profile_counters() { return text(brp_hip_profile_counters); }
 
-    // Batched extension (fused engine): another observation on the same handle; templates
-    // name it by (n, unpadded).
+    // Extensions (fused engine): the C handle, another observation on it (templates name
+    // their series by (n, unpadded)), and the bank loop over prepared templates of any
+    // registered series with one C toplist per series; returns each template's candidate
+    // count.
+    brp_hip* handle() const { return handle_.get(); }
     void add_series(const Params& p, std::span<const float> input) {
-        check(brp_hip_add_series(impl_.get(), &p, input.data(), input.size()));
+        hip_detail::check(brp_hip_add_series(handle_.get(), hip_detail::c_params(&p), input.data(), input.size()));
     }
-    // src/workunit_main.cpp:77-136 over prepared templates of any registered series, one
-    // toplist per series (registration order); returns the candidate count per template.
     std::vector<uint32_t> bank_search(std::span<const Params> templates, std::span<const brp_orbit> orbits,
                                       std::span<brp_toplist* const> tops) {
         if (templates.size() != orbits.size()) throw std::runtime_error("templates and orbits differ in length");
         std::vector<uint32_t> counts(templates.size());
         double gpu_ms = 0;
-        check(brp_hip_bank_search_templates(impl_.get(), templates.data(), orbits.data(), templates.size(),
-                                            tops.data(), tops.size(), counts.data(), &gpu_ms));
+        hip_detail::check(brp_hip_bank_search_templates(handle_.get(), hip_detail::c_params(templates.data()),
+                                                        orbits.data(), templates.size(), tops.data(), tops.size(),
+                                                        counts.data(), &gpu_ms));
         return counts;
     }
 
 private:
-    struct Free { void operator()(brp_hip* h) const { brp_hip_destroy(h); } };
-    std::unique_ptr<brp_hip, Free> impl_;
-    Params p_;
-    uint32_t batch_;
-    std::vector<Params> templates_;
+    static brp_options reference_options(uint32_t workgroup, uint32_t tile, uint32_t batch, bool software,
+                                         bool stockham, bool radix4, bool packed, bool scatter, bool radix8,
+                                         bool fused_power, bool radix8_store) {
+        brp_options o{};
+        o.workgroup = workgroup;
+        o.tile = tile;
+        o.batch = batch;
+        o.software = software;
+        o.stockham = stockham;
+        o.radix4 = radix4;
+        o.packed = packed;
+        o.scatter = scatter;
+        o.radix8 = radix8;
+        o.fused_power = fused_power;
+        o.radix8_store = radix8_store;
+        o.engine = BRP_ENGINE_PLAIN;
+        return o;
+    }
+    template <class F>
+    std::string text(F&& read) const {
+        std::string s(read(handle_.get(), nullptr, 0) + 1, '\0');
+        read(handle_.get(), s.data(), s.size());
+        s.pop_back();
+        return s;
+    }
+    struct Destroy { void operator()(brp_hip* h) const { brp_hip_destroy(h); } };
+    std::unique_ptr<brp_hip, Destroy> handle_;
+    std::vector<Params> current_;
 };
-
-// load_observation's decode + preprocess (src/workload.cpp:29-95) on the device, for the
-// sample block of a DD file; returns the preprocessed series (samples floats).
-inline std::vector<float> preprocess(std::span<const uint8_t> block, size_t samples, uint32_t bits, double scale,
-                                     uint32_t n, uint32_t window = 1000, float rfi_ratio = 30.0f,
-                                     uint32_t* rfi_bins = nullptr) {
-    std::vector<float> out(samples);
-    check(brp_hip_preprocess(block.data(), samples, bits, scale, n, window, rfi_ratio, out.data(), nullptr, rfi_bins));
-    return out;
-}
-
-// The same with the reference's zap list (src/workload.cpp:76-86): (lower, upper) Hz
-// pairs flattened; decoded (may be null) receives the decoded samples.
-inline std::vector<float> preprocess(std::span<const uint8_t> block, size_t samples, uint32_t bits, double scale,
-                                     uint32_t n, uint32_t window, float rfi_ratio, std::span<const double> zap,
-                                     double sample_us, double padding, std::vector<float>* decoded = nullptr,
-                                     uint32_t* rfi_bins = nullptr) {
-    std::vector<float> out(samples);
-    if (decoded) decoded->resize(samples);
-    check(brp_hip_preprocess_zap(block.data(), samples, bits, scale, n, window, rfi_ratio, zap.data(), zap.size() / 2,
-                                 sample_us, padding, out.data(), decoded ? decoded->data() : nullptr, rfi_bins));
-    return out;
-}
 
 }  // namespace brp
