@@ -55,6 +55,15 @@ __device__ __forceinline__ float fma_sin(float phase, const float2* __restrict__
     return __builtin_fmaf(-d2, sc.x, __builtin_fmaf(d, sc.y, sc.x));
 }
 
+// The orbital offset of sample i in samples (src/science.cpp:36-40), rounding as the host
+// rule brp_sample_offset does: the nested-FMA form with flag 2, else separate roundings.
+__device__ __forceinline__ float sample_offset(uint32_t i, const brp_params& p, const float2* __restrict__ lut) {
+    const float t = float(i) * p.dt;
+    if (p.flags & 2u)
+        return __builtin_fmaf(p.tau * fma_sin(__builtin_fmaf(p.omega, t, p.psi), lut), p.step_inv, -p.s0);
+    return p.tau * lut_sin(p.omega * t + p.psi, lut) * p.step_inv - p.s0;
+}
+
 struct Orbit32 { float tau, omega, psi, dt, step_inv, s0; uint32_t flags; };
 
 // Source index of resampled sample i: int(double(float(i) - offset(i)) + 0.5), the

@@ -92,6 +92,10 @@ std::vector<FusedShape> fused_shapes(uint32_t n);
 void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a);
 void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a);
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles);
+// resampling_steps of every template of params[0 .. count) whose steps is 0, on the
+// device (the host's jump-started scan, for templates where it is safe); error bit 1 when
+// one produces an empty series.
+void search_steps(hipStream_t st, brp_params* params, uint32_t count, const float2* lut, uint32_t* error);
 // bytes / 16 uint4 copies from src to dst (a multiple of 16 bytes).
 void copy16(hipStream_t st, const void* src, void* dst, size_t bytes);
 // --------------------------------------------------------- device toplist --

@@ -170,6 +170,13 @@ struct brp_hip {
     uint32_t* slot_table_host = nullptr;  // pinned, 2 * batch
     float4* slot_orbit_host = nullptr;    // pinned, 2 * batch
 
+    // Templates whose resampled length the device computes (brp_hip_bank_search on the fused
+    // engine leaves their steps at 0): per half, whether the batch has any, and the error
+    // word of the length kernel (device and pinned host copies).
+    bool deferred[2] = {false, false};
+    uint32_t* steps_error_dev = nullptr;
+    uint32_t* steps_error_host = nullptr;
+
     size_t slot_base(int half) const { return size_t(half) * opt.batch; }
 
     template <class T>
@@ -217,6 +224,7 @@ struct brp_hip {
         if (params_host) hipHostFree(params_host);
         if (slot_table_host) hipHostFree(slot_table_host);
         if (slot_orbit_host) hipHostFree(slot_orbit_host);
+        if (steps_error_host) hipHostFree(steps_error_host);
         for (hipEvent_t e : kernels_done) if (e) hipEventDestroy(e);
         for (hipEvent_t e : retained) if (e) hipEventDestroy(e);
         if (top_stream) hipStreamDestroy(top_stream);
@@ -383,6 +391,9 @@ void create_fused(brp_hip& h) {
     h.stride = h.opt.timing_stride ? h.opt.timing_stride : 1;
     auto& a = h.fa;
     a.mean = h.alloc<float>(2 * size_t(h.opt.batch));
+    h.steps_error_dev = h.alloc<uint32_t>(2);
+    HIPC(hipHostMalloc(reinterpret_cast<void**>(&h.steps_error_host), 2 * sizeof(uint32_t), hipHostMallocDefault));
+    h.steps_error_host[0] = h.steps_error_host[1] = 0;
     a.results = h.results;
     a.slot_words = h.slot_words;
     a.params = h.params_dev;
@@ -406,11 +417,11 @@ uint32_t series_of(const brp_hip& h, const brp_params& p) {
 // The fixed-dimension rule of src/vulkan.cpp:270-276, lifted for the fused engine: a
 // template may name any registered series (its n and unpadded), with its own F and H;
 // the result capacity and the arithmetic flags stay those of the handle.
-void check_template(const brp_hip& h, const brp_params& next) {
+void check_template(const brp_hip& h, const brp_params& next, bool steps_deferred = false) {
     if (h.opt.engine == BRP_ENGINE_PLAIN) return check_template(h.base, next);
     require((next.flags & 6) == (h.base.flags & 6), "arithmetic specialization changed; rebuild pipelines");
     (void)series_of(h, next);
-    require(next.capacity == h.base.capacity && next.steps && next.steps <= next.unpadded,
+    require(next.capacity == h.base.capacity && (next.steps || steps_deferred) && next.steps <= next.unpadded,
             "template dimensions changed; rebuild resources");
     require(next.window <= next.fundamental && next.harmonic <= next.n / 2 && next.fundamental <= next.n / 2,
             "template search range outside the spectrum");
@@ -612,6 +623,10 @@ void launch(brp_hip& h, int half) {
     if (h.device_top && h.retain_pending[half]) HIPC(hipStreamWaitEvent(h.stream, h.retained[half], 0));
     HIPC(hipMemcpyAsync(h.params_dev + base, h.params_host + base, sizeof(brp_params) * count,
                         hipMemcpyHostToDevice, h.stream));
+    if (h.deferred[half]) {
+        HIPC(hipMemsetAsync(h.steps_error_dev + half, 0, sizeof(uint32_t), h.stream));
+        brp_engine::search_steps(h.stream, h.params_dev + base, count, h.lut, h.steps_error_dev + half);
+    }
     if (h.device_top) {
         HIPC(hipMemcpyAsync(h.slot_table_dev + base, h.slot_table_host + base, sizeof(uint32_t) * count,
                             hipMemcpyHostToDevice, h.stream));
@@ -650,6 +665,12 @@ void launch(brp_hip& h, int half) {
     h.submit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - submit0).count();
     HIPC(hipMemcpyAsync(h.staging_host + base * (h.keep + 1) * 4, h.staging_dev + base * (h.keep + 1) * 4,
                         sizeof(uint32_t) * size_t(count) * (h.keep + 1) * 4, hipMemcpyDeviceToHost, h.stream));
+    if (h.deferred[half]) {  // the lengths as run, and the length kernel's error word
+        HIPC(hipMemcpyAsync(h.params_host + base, h.params_dev + base, sizeof(brp_params) * count,
+                            hipMemcpyDeviceToHost, h.stream));
+        HIPC(hipMemcpyAsync(h.steps_error_host + half, h.steps_error_dev + half, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, h.stream));
+    }
     if (!h.done[half]) HIPC(hipEventCreateWithFlags(&h.done[half], hipEventDisableTiming));
     HIPC(hipEventRecord(h.done[half], h.stream));
     h.ran[half] = false;
@@ -659,6 +680,11 @@ void finish(brp_hip& h, int half) {
     HIPC(hipEventSynchronize(h.done[half]));
     h.ran[half] = true;
     const uint32_t count = uint32_t(h.templates[half].size());
+    if (h.deferred[half]) {
+        require(!h.steps_error_host[half], "template produces empty series");
+        const size_t base = h.slot_base(half);
+        for (uint32_t k = 0; k < count; ++k) h.templates[half][k].steps = h.params_host[base + k].steps;
+    }
     fill_timings(h, half, count);
     for (int s = 0; s < 4; ++s) h.total_ms[s] += h.kernel_ms[half][s];
     h.total_launches += h.timed_launches[half];
@@ -1062,11 +1088,12 @@ namespace {
 // and the launch plans repeat, so captured graphs are replayed.  Thresholds are frozen
 // at each batch start and retain() replays template order within each series.
 void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit* orbits, brp_toplist* const* tops,
-               size_t ntops, uint32_t* cand_count, brp_params* templates_out, double* gpu_ms) {
+               size_t ntops, uint32_t* cand_count, brp_params* templates_out, double* gpu_ms,
+               bool steps_deferred = false) {
     const size_t count = base.size();
     std::vector<uint32_t> ser(count, 0);
     for (size_t i = 0; i < count; ++i) {
-        check_template(h, base[i]);
+        check_template(h, base[i], steps_deferred);
         if (h.opt.engine == BRP_ENGINE_FUSED) ser[i] = series_of(h, base[i]);
         require(ser[i] < ntops && tops[ser[i]], "no toplist for a template's series");
     }
@@ -1085,9 +1112,11 @@ void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit*
         const int half = overlap ? int(b & 1) : 0;
         const size_t first = b * batch, n = std::min(batch, count - first);
         h.templates[half].resize(n);
+        h.deferred[half] = false;
         for (size_t k = 0; k < n; ++k) {
             const size_t i = seq[first + k];
             h.templates[half][k] = base[i];
+            h.deferred[half] = h.deferred[half] || !base[i].steps;
             brp_toplist_thresholds(tops[ser[i]], &h.templates[half][k]);
         }
         launch(h, half);
@@ -1104,6 +1133,7 @@ void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit*
         for (size_t k = 0; k < n; ++k) {
             const size_t i = seq[first + k];
             brp_params now = base[i];
+            now.steps = h.templates[half][k].steps;  // as run (computed on the device when deferred)
             brp_toplist* top = tops[ser[i]];
             brp_toplist_thresholds(top, &now);  // the toplist as of this template
             read_slot(h, half, uint32_t(k), cand);
@@ -1114,6 +1144,7 @@ void bank_loop(brp_hip& h, const std::vector<brp_params>& base, const brp_orbit*
         }
         if (!overlap && b + 1 < batches) enqueue(b + 1);
     }
+    h.deferred[0] = h.deferred[1] = false;
     if (gpu_ms) *gpu_ms = total_gpu;
 }
 
@@ -1344,13 +1375,44 @@ int brp_hip_bank_search(brp_hip* h, const brp_orbit* orbits, size_t count, uint3
                         uint32_t* cand_count, double* gpu_ms) {
     return guarded([&] {
         require(h && orbits && top, "null argument");
-        // Parameters that do not depend on the toplist, once per template.
+        // Parameters that do not depend on the toplist, once per template.  The plain engine
+        // reads them on the host, so it takes brp_search_parameters whole.  On the fused
+        // engine the host sets the per-template fields (src/workload.cpp:109-112, with the
+        // host's sinf in S0) and the per-length ones once (window, F, H, thresholds,
+        // src/workload.cpp:114-124), and the resampled lengths (resampling_steps) are left to
+        // the device, in the launch stream of each batch; templates whose delay is too long
+        // for the jump-started scan keep the host's scan.
         std::vector<brp_params> base(count);
-        for (size_t i = 0; i < count; ++i)
-            if (int rc = brp_search_parameters(h->base.n, samples, sample_us, &orbits[i], f0, false_alarm, window, &base[i]))
+        const bool device_steps = h->opt.engine == BRP_ENGINE_FUSED;
+        brp_params limits{};
+        for (size_t i = 0; i < count; ++i) {
+            brp_params& p = base[i];
+            if (!device_steps) {
+                if (int rc = brp_search_parameters(h->base.n, samples, sample_us, &orbits[i], f0, false_alarm, window, &p))
+                    throw Failure(rc, brp_last_error());
+                continue;
+            }
+            if (int rc = brp::search_fields(h->base.n, samples, sample_us, orbits[i], f0, false_alarm, window, &p))
                 throw Failure(rc, brp_last_error());
+            if (!brp::steps_jump_start_safe(p)) {
+                brp_params plain = p;
+                plain.flags = 0;
+                plain.s0 = brp::double_sine_s0(p);
+                if (int rc = brp_resampling_steps(&plain)) throw Failure(rc, brp_last_error());
+                if (int rc = brp_resampling_steps(&p)) throw Failure(rc, brp_last_error());
+            }
+            if (i == 0) {
+                limits = p;
+                if (int rc = brp::search_limits(sample_us, f0, false_alarm, window, &limits))
+                    throw Failure(rc, brp_last_error());
+            }
+            p.window = limits.window;
+            p.fundamental = limits.fundamental;
+            p.harmonic = limits.harmonic;
+            std::copy(limits.threshold, limits.threshold + 5, p.threshold);
+        }
         brp_toplist* tops[1] = {top};
-        bank_loop(*h, base, orbits, tops, 1, cand_count, templates_out, gpu_ms);
+        bank_loop(*h, base, orbits, tops, 1, cand_count, templates_out, gpu_ms, device_steps);
     });
 }
 

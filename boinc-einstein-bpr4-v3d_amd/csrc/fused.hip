@@ -672,6 +672,52 @@ __global__ void k_pack_results(const uint32_t* __restrict__ results, uint32_t sl
     for (uint32_t e = threadIdx.x; e <= n; e += blockDim.x) dst[e] = src[e];
 }
 
+// ------------------------------------------------------ resampled lengths --
+// resampling_steps (src/science.cpp:41-45) of a template whose jump start is safe: the
+// largest steps <= unpadded - 1 with float(steps) - offset(steps) < unpadded - 1 in the FP32
+// rule, by the host's method: a scan down from 64 samples above the FP64 solution s* of
+// s - offset(s) = unpadded - 1 (a fixed point with contraction factor tau omega < 1/2).  The
+// host only defers templates for which no s more than 64 samples above s* can satisfy the
+// rule, so the scan's result is the reference's; the device FP64 sine moves s* by far less
+// than a sample.
+__device__ uint32_t resampled_length(const brp_params& p, const float2* __restrict__ lut) {
+    const float last = float(p.unpadded - 1);
+    uint32_t steps = p.unpadded - 1;
+    const double u1 = double(p.unpadded - 1);
+    double s = u1;
+    for (int it = 0; it < 200; ++it) {
+        const double next =
+            u1 + (double(p.tau) * sin_f64(double(p.omega) * (s * double(p.dt)) + double(p.psi)) * double(p.step_inv) -
+                  double(p.s0));
+        const bool done = fabs(next - s) < 1e-3;
+        s = next;
+        if (done) break;
+    }
+    const double start = floor(s) + 64.0;
+    if (start < double(steps)) steps = start < 1.0 ? 1u : uint32_t(start);
+    while (steps && float(steps) - sample_offset(steps, p, lut) >= last) --steps;
+    return steps;
+}
+
+// The lengths of the templates the host left at steps = 0, in place: first the length
+// parameters() computes with the double-sine S0 and plain arithmetic, whose emptiness is an
+// error as in the reference (src/workload.cpp:110), then the template's own (flag-2
+// arithmetic, single-sine S0).  error bit 1: a template produces an empty series.
+__global__ __launch_bounds__(64) void k_search_steps(brp_params* params, uint32_t count,
+                                                     const float2* __restrict__ lut, uint32_t* error) {
+    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= count) return;
+    brp_params p = params[t];
+    if (p.steps) return;
+    brp_params plain = p;
+    plain.flags = 0;
+    plain.s0 = float(double(p.tau) * sin_f64(double(p.psi)) * double(p.step_inv));
+    const uint32_t first = resampled_length(plain, lut);
+    const uint32_t steps = resampled_length(p, lut);
+    if (!first || !steps) atomicOr(error, 1u);
+    params[t].steps = steps;
+}
+
 // Plain streaming copy, 16 bytes per lane per iteration (the counter calibration kernel).
 __global__ __launch_bounds__(256) void brp_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
@@ -847,6 +893,10 @@ void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
 void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles) {
     if (a.snap_summed) k_harmonic<true><<<tiles * a.count, kHarmThreads, 0, st>>>(a, s, tiles);
     else k_harmonic<false><<<tiles * a.count, kHarmThreads, 0, st>>>(a, s, tiles);
+}
+
+void search_steps(hipStream_t st, brp_params* params, uint32_t count, const float2* lut, uint32_t* error) {
+    k_search_steps<<<(count + 63) / 64, 64, 0, st>>>(params, count, lut, error);
 }
 
 void copy16(hipStream_t st, const void* src, void* dst, size_t bytes) {

@@ -98,6 +98,87 @@ uint64_t mix64(uint64_t z) {
 
 }  // namespace
 
+namespace brp {
+
+// The fields of src/science.cpp:46-65 that do not need the resampled length: validation,
+// FFT shape, tau, omega, psi, dt, 1/dt and the double-sine S0 (steps left 0).
+int template_fields(uint32_t n, uint32_t unpadded, float tau, float period, float psi, float dt, brp_params* out) {
+    const uint32_t radix = is_pow2(n) ? 1 : ((n % 3 == 0 && is_pow2(n / 3)) ? 3 : 0);
+    if (!radix || n < 64 || n > 3u * (1u << 22) || unpadded < 32 || unpadded > n || dt <= 0 ||
+        period <= 0 || tau < 0 || !std::isfinite(tau + period + psi + dt))
+        return brp::fail(BRP_E_INVALID, "invalid observation/template");
+    brp_params p{};
+    p.n = n;
+    p.unpadded = unpadded;
+    p.fft_n = n;
+    p.fft_radix = radix;
+    p.fft_base = n / radix;
+    p.logn = uint32_t(__builtin_ctz(p.fft_base));
+    p.tau = tau;
+    p.omega = float(kTwoPi / double(period));
+    p.psi = psi;
+    p.dt = dt;
+    p.step_inv = 1.0f / dt;
+    // The historical driver takes S0 from a double sine, the resampler a LUT sine.
+    p.s0 = double_sine_s0(p);
+    if (!std::isfinite(p.omega) || !std::isfinite(p.step_inv) || !std::isfinite(p.s0) ||
+        !std::isfinite(p.tau * p.step_inv) || !std::isfinite(float(unpadded) * p.dt) ||
+        !std::isfinite(p.omega * (float(unpadded) * p.dt) + p.psi))
+        return brp::fail(BRP_E_INVALID, "template exceeds finite FP32 arithmetic domain");
+    *out = p;
+    return BRP_OK;
+}
+
+float double_sine_s0(const brp_params& p) {
+    return float(double(p.tau) * std::sin(double(p.psi)) * double(p.step_inv));
+}
+
+// src/workload.cpp:109-112 without the resampled length (steps left 0): the search
+// parameters' check, the template fields, the FMA arithmetic flags and the
+// single-precision-sine S0.  The double-sine length parameters() computes first
+// (src/workload.cpp:110) is the caller's to check.
+int search_fields(uint32_t n, uint32_t samples, double sample_us, const brp_orbit& orbit, double f0,
+                  double false_alarm, uint32_t window, brp_params* out) {
+    if (!std::isfinite(f0 + false_alarm) || f0 <= 0 || false_alarm <= 0 || false_alarm >= 1 || !window)
+        return brp::fail(BRP_E_INVALID, "invalid search parameters");
+    brp_params p{};
+    if (int rc = template_fields(n, samples, float(orbit.tau), float(orbit.period), float(orbit.psi),
+                                 float(sample_us * 1e-6), &p))
+        return rc;
+    p.capacity = 65536;  // parameters()' result capacity (src/science.cpp:46-65)
+    p.flags = 2 | 4;     // production FMA arithmetic
+    p.s0 = (p.tau * std::sin(p.psi)) * p.step_inv;  // single-precision sine here
+    *out = p;
+    return BRP_OK;
+}
+
+// src/workload.cpp:114-124: window, F, H and the chi^2 thresholds, which depend on the
+// series length and the search settings only.
+int search_limits(double sample_us, double f0, double false_alarm, uint32_t window, brp_params* p) {
+    const double seconds = double(p->n) * sample_us * 1e-6;
+    p->window = uint32_t(window * 0.5 + 0.5);
+    if (p->window >= p->n / 2 + 1) return brp::fail(BRP_E_INVALID, "search window exceeds spectrum");
+    const double limit = double(p->n / 2 + 1 - p->window);
+    p->fundamental = uint32_t(std::min(limit, f0 * seconds + 0.5));
+    p->harmonic = uint32_t(std::min(limit, 16 * f0 * seconds + 0.5));
+    const float probability = float(1.0 - std::pow(1.0 - false_alarm, 1.0 / double(p->n / 2 + 1)));
+    if (!(probability > 0 && probability < 1))
+        return brp::fail(BRP_E_INVALID, "false-alarm probability outside representable domain");
+    for (uint32_t h = 0; h < 5; ++h)
+        p->threshold[h] = float(0.5 * upper_tail_quantile(double(probability), 1u << h));
+    if (p->fundamental <= p->window) return brp::fail(BRP_E_INVALID, "empty frequency search");
+    return BRP_OK;
+}
+
+bool steps_jump_start_safe(const brp_params& p) {
+    const double delay = std::fabs(double(p.tau) * double(p.step_inv));  // tau/dt in samples
+    const double max_phase = std::fabs(double(p.omega)) * double(p.unpadded) * std::fabs(double(p.dt)) +
+                             std::fabs(double(p.psi));
+    return double(p.tau) * double(p.omega) < 0.5 && delay * (2.5e-5 + 1e-6 * (1.0 + max_phase)) <= 15.0;
+}
+
+}  // namespace brp
+
 extern "C" {
 
 float brp_lut_sin(float phase) { return lut_sin(phase); }
@@ -124,11 +205,7 @@ int brp_resampling_steps(brp_params* p) {
     if (!p || p->unpadded < 2) return brp::fail(BRP_E_INVALID, "invalid template");
     const float last = float(p->unpadded - 1);
     uint32_t steps = p->unpadded - 1;
-    const double delay = std::fabs(double(p->tau) * double(p->step_inv));  // tau/dt in samples
-    const double max_phase = std::fabs(double(p->omega)) * double(p->unpadded) * std::fabs(double(p->dt)) +
-                             std::fabs(double(p->psi));
-    const double offset_error = delay * (2.5e-5 + 1e-6 * (1.0 + max_phase));
-    if (double(p->tau) * double(p->omega) < 0.5 && offset_error <= 15.0) {
+    if (brp::steps_jump_start_safe(*p)) {
         auto offset = [&](double s) {
             return double(p->tau) * std::sin(double(p->omega) * (s * double(p->dt)) + double(p->psi)) *
                        double(p->step_inv) - double(p->s0);
@@ -153,28 +230,8 @@ int brp_resampling_steps(brp_params* p) {
 int brp_parameters(uint32_t n, uint32_t unpadded, float tau, float period, float psi, float dt,
                    brp_params* out) {
     if (!out) return brp::fail(BRP_E_INVALID, "null output");
-    const uint32_t radix = is_pow2(n) ? 1 : ((n % 3 == 0 && is_pow2(n / 3)) ? 3 : 0);
-    if (!radix || n < 64 || n > 3u * (1u << 22) || unpadded < 32 || unpadded > n || dt <= 0 ||
-        period <= 0 || tau < 0 || !std::isfinite(tau + period + psi + dt))
-        return brp::fail(BRP_E_INVALID, "invalid observation/template");
     brp_params p{};
-    p.n = n;
-    p.unpadded = unpadded;
-    p.fft_n = n;
-    p.fft_radix = radix;
-    p.fft_base = n / radix;
-    p.logn = uint32_t(__builtin_ctz(p.fft_base));
-    p.tau = tau;
-    p.omega = float(kTwoPi / double(period));
-    p.psi = psi;
-    p.dt = dt;
-    p.step_inv = 1.0f / dt;
-    // The historical driver takes S0 from a double sine, the resampler a LUT sine.
-    p.s0 = float(double(tau) * std::sin(double(psi)) * double(p.step_inv));
-    if (!std::isfinite(p.omega) || !std::isfinite(p.step_inv) || !std::isfinite(p.s0) ||
-        !std::isfinite(p.tau * p.step_inv) || !std::isfinite(float(unpadded) * p.dt) ||
-        !std::isfinite(p.omega * (float(unpadded) * p.dt) + p.psi))
-        return brp::fail(BRP_E_INVALID, "template exceeds finite FP32 arithmetic domain");
+    if (int rc = brp::template_fields(n, unpadded, tau, period, psi, dt, &p)) return rc;
     if (int rc = brp_resampling_steps(&p)) return rc;
     p.harmonic = n / 2;
     p.fundamental = n / 2;
@@ -189,27 +246,16 @@ int brp_parameters(uint32_t n, uint32_t unpadded, float tau, float period, float
 int brp_search_parameters(uint32_t n, uint32_t samples, double sample_us, const brp_orbit* orbit,
                           double f0, double false_alarm, uint32_t window, brp_params* out) {
     if (!orbit || !out) return brp::fail(BRP_E_INVALID, "null argument");
-    if (!std::isfinite(f0 + false_alarm) || f0 <= 0 || false_alarm <= 0 || false_alarm >= 1 || !window)
-        return brp::fail(BRP_E_INVALID, "invalid search parameters");
     brp_params p{};
-    if (int rc = brp_parameters(n, samples, float(orbit->tau), float(orbit->period), float(orbit->psi),
-                                float(sample_us * 1e-6), &p))
-        return rc;
-    p.flags = 2 | 4;  // production FMA arithmetic
-    p.s0 = (p.tau * std::sin(p.psi)) * p.step_inv;  // single-precision sine here
+    if (int rc = brp::search_fields(n, samples, sample_us, *orbit, f0, false_alarm, window, &p)) return rc;
+    // parameters() computes the length with the double-sine S0 first (src/workload.cpp:110).
+    brp_params first = p;
+    first.flags = 0;
+    first.s0 = brp::double_sine_s0(p);
+    if (int rc = brp_resampling_steps(&first)) return rc;
+    // Then with the FMA arithmetic and the single-precision sine (src/workload.cpp:111-113).
     if (int rc = brp_resampling_steps(&p)) return rc;
-    const double seconds = double(n) * sample_us * 1e-6;
-    p.window = uint32_t(window * 0.5 + 0.5);
-    if (p.window >= p.n / 2 + 1) return brp::fail(BRP_E_INVALID, "search window exceeds spectrum");
-    const double limit = double(p.n / 2 + 1 - p.window);
-    p.fundamental = uint32_t(std::min(limit, f0 * seconds + 0.5));
-    p.harmonic = uint32_t(std::min(limit, 16 * f0 * seconds + 0.5));
-    const float probability = float(1.0 - std::pow(1.0 - false_alarm, 1.0 / double(p.n / 2 + 1)));
-    if (!(probability > 0 && probability < 1))
-        return brp::fail(BRP_E_INVALID, "false-alarm probability outside representable domain");
-    for (uint32_t h = 0; h < 5; ++h)
-        p.threshold[h] = float(0.5 * upper_tail_quantile(double(probability), 1u << h));
-    if (p.fundamental <= p.window) return brp::fail(BRP_E_INVALID, "empty frequency search");
+    if (int rc = brp::search_limits(sample_us, f0, false_alarm, window, &p)) return rc;
     *out = p;
     return BRP_OK;
 }

@@ -365,6 +365,32 @@ def _oracle_bank(oracle, x, templates, orbits, rows=100):
     return top
 
 
+def test_bank_search_device_lengths_equal_host_parameters(brp, wl):
+    """brp_hip_bank_search on the fused engine leaves the resampled lengths to the device
+    (resampling_steps in the launch stream of each batch).  The parameter blocks it ran equal
+    brp_search_parameters byte for byte, apart from the thresholds the toplist raised: 1024 seeded bank orbits at N = 2^20 and 2^22 plus
+    edge orbits (no delay, psi < 0 and > 2 pi, a tiny delay, a delay of 32 s, and delays of
+    40, 60 and 150 s, which exceed the jump start's error bound and keep the host's scan; all
+    with every resampled index inside the series)."""
+    edge = [(3600.0, 0.0, 1.0), (660.0, 0.3, 6.2831), (5000.0, 0.2, -1.0), (5000.0, 0.2, 7.5), (700.0, 1e-6, 3.0),
+            (20000.0, 32.0, 2.0), (40000.0, 40.0, 0.5), (80000.0, 60.0, 0.3), (200000.0, 150.0, 0.1)]
+    for n in (1 << 20, 1 << 22):
+        w = wl.bank(count=1024 - len(edge), n=n)
+        orbits = edge + list(w.orbits)
+        say(f"[gpu] bank_search lengths on the device, N={n}, {len(orbits)} orbits")
+        with brp.GPU(w.templates[0], w.x, 256, 0, 128, packed=True, engine=brp.ENGINE_FUSED, graph=True,
+                     untimed=True) as g:
+            ran, counts, _ = g.bank_search(orbits, n, wl.SAMPLE_US, brp.Toplist(100))
+        for o, p in zip(orbits, ran):
+            want = brp.search_parameters(n, n, wl.SAMPLE_US, o)
+            # The blocks come back as run: thresholds raised by the toplist, never lowered.
+            assert all(p.threshold[h] >= want.threshold[h] for h in range(5)), o
+            for h in range(5):
+                want.threshold[h] = p.threshold[h]
+            assert bytes(p) == bytes(want), o
+        assert counts.sum() > 0
+
+
 def test_bank_toplist_identical(brp, oracle, wl):
     count, n = 24, 1 << 20
     w = wl.bank(count=count, n=n)
