@@ -43,7 +43,6 @@ struct FusedShape {
     uint32_t s1 = 0, s2 = 0;              // rows per pass-1 block, lower columns per pass-2 block
     uint32_t t1 = 0, t2 = 0;              // threads per pass-1 / pass-2 block
     uint32_t blocks1 = 0, blocks2 = 0;    // blocks per template
-    uint32_t tile = 1024;                 // harmonic bins per block
     uint32_t mirror = 0;                  // exchange rows with each column beside its mirror
 };
 
@@ -70,6 +69,7 @@ struct FusedArgs {
     uint32_t fma = 0;                // flag 2 arithmetic (uniform over a handle)
     uint32_t nonneg = 0;             // every template of the group has omega, psi, dt >= 0
     uint32_t precision = 0;          // 1: pass-2 radix-16 stages as split-FP16 MFMA GEMMs
+    uint32_t power_bins = 0;         // pass 2 writes power bins [0, power_bins) only (0: all M)
     // snapshot (per batch slot, or null)
     float* snap_resampled = nullptr;  // N floats
     int32_t* snap_index = nullptr;    // N int32
@@ -91,7 +91,12 @@ std::vector<FusedShape> fused_shapes(uint32_t n);
 // One launch each covers all `count` templates of the group.
 void fused_pass1(hipStream_t st, const FusedShape& s, const FusedArgs& a);
 void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a);
-void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles);
+// The harmonic kernel over bins [0, reach) of every template of the group (reach: the bins
+// the fundamentals below F need, at most min(16 F, H + 8)).
+void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t reach);
+// The power bins [0, result) the harmonic kernel reads for that reach (whole tiles): the
+// FusedArgs::power_bins a group without snapshots passes to pass 2.
+uint32_t harmonic_power_bins(uint32_t reach);
 // resampling_steps of every template of params[0 .. count) whose steps is 0, on the
 // device (the host's jump-started scan, for templates where it is safe); error bit 1 when
 // one produces an empty series.

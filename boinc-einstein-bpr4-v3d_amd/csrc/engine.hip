@@ -129,10 +129,10 @@ struct brp_hip {
     uint32_t m_max = 0, blocks1_max = 0, n_max = 0;   // what the shared buffers hold
     // A launch group: `count` consecutive slots from `first`, one series, one index form.
     struct Group {
-        uint32_t first, count, series, nonneg, tiles;
+        uint32_t first, count, series, nonneg, reach;  // reach: harmonic bins to cover
         bool operator==(const Group& o) const {
             return first == o.first && count == o.count && series == o.series && nonneg == o.nonneg &&
-                   tiles == o.tiles;
+                   reach == o.reach;
         }
     };
     std::vector<Group> groups[2];    // of the last launch of each half
@@ -467,7 +467,7 @@ std::vector<brp_hip::Group> plan_groups(const brp_hip& h, int half) {
             nn = nn && tp[i].omega >= 0.0f && tp[i].psi >= 0.0f && tp[i].dt >= 0.0f;
             reach = std::max(reach, std::min<uint64_t>(16ull * tp[i].fundamental, uint64_t(tp[i].harmonic) + 8));
         }
-        out.push_back({first, end - first, s, nn ? 1u : 0u, uint32_t((reach + 1023) / 1024)});
+        out.push_back({first, end - first, s, nn ? 1u : 0u, uint32_t(reach)});
         first = end;
     }
     return out;
@@ -521,12 +521,13 @@ void enqueue_fused(brp_hip& h, int half, bool capturing) {
         a.count = grp.count;
         a.fma = (h.base.flags & 2u) ? 1u : 0u;
         a.nonneg = grp.nonneg;
+        a.power_bins = h.opt.snapshot ? 0u : brp_engine::harmonic_power_bins(grp.reach);
         record(4 * gi);
         brp_engine::fused_pass1(st, ser.shape, a);
         record(4 * gi + 1);
         brp_engine::fused_pass2(st, ser.shape, a);
         record(4 * gi + 2);
-        brp_engine::fused_harmonic(st, ser.shape, a, grp.tiles);
+        brp_engine::fused_harmonic(st, ser.shape, a, grp.reach);
         record(4 * gi + 3);
     }
     if (h.lanes == 2) {

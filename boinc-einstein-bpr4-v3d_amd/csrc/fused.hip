@@ -13,10 +13,11 @@
 //           X[k] = 1/2 [(Z_k + conj Z_{M-k}) - i W_N^k (Z_k - conj Z_{M-k})], the mean pad
 //           added by linearity, X += mu U with U[k] = sum_{t >= steps} W_N^{tk}
 //           = (W_N^{k steps} - 1) / (1 - W_N^k), and the power |X|^2 / N.
-//   harmonic (block = 1024 bins + 8 halo):  dyadic sums of every bin into LDS, then the
-//           maximum over each fundamental's rounding interval, threshold test and
-//           candidate append (the gather result of shaders/harmonic.comp:14-36 without
-//           recomputing sums per level).
+//   harmonic (block = one wave, 16 bins per lane):  dyadic sums of the lane's bins in
+//           registers from one run of consecutive powers per harmonic, then the maximum over
+//           each fundamental's rounding interval (8 sums from the left lane by a wave shift),
+//           threshold test and candidate append (the gather result of
+//           shaders/harmonic.comp:14-36 without recomputing sums per level).
 //
 // One launch of each kernel covers every template of a group, blocks mapped so that
 // consecutive work items stay on one XCD (8 XCDs, block b runs on XCD b mod 8).
@@ -31,8 +32,6 @@ using brp_engine::FusedShape;
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kTile = 1024;  // harmonic bins per block
 
 struct Radices { int count; int r[5]; };
 
@@ -446,6 +445,8 @@ k_pass2(FusedArgs a, FusedShape sh) {
     __syncthreads();
 
     float* P = a.P + size_t(g) * M;
+    // Power bins past the harmonic kernel's reach are not written (snapshots keep them all).
+    const uint32_t lim = (SNAP || a.power_bins == 0) ? uint32_t(M) : a.power_bins;
     float2* snapX = SNAP ? a.snap_X + size_t(slot) * a.snap_m1 : nullptr;
     const float inv_n = 1.0f / float(N);
     const uint32_t steps = p.steps;
@@ -509,8 +510,8 @@ k_pass2(FusedArgs a, FusedShape sh) {
         const cf zm = load(lds + lds_at<SS>(int(k1m), ms));
         cf xk, xm;
         real_split(zk, zm, Wk[u], we, k, xk, xm);
-        P[k] = k == 0 ? 0.0f : (xk.x * xk.x + xk.y * xk.y) * inv_n;
-        if (!self) P[M - k] = (xm.x * xm.x + xm.y * xm.y) * inv_n;
+        if (k < lim) P[k] = k == 0 ? 0.0f : (xk.x * xk.x + xk.y * xk.y) * inv_n;
+        if (!self && M - k < lim) P[M - k] = (xm.x * xm.x + xm.y * xm.y) * inv_n;
         if constexpr (SNAP) {
             store(snapX + k, xk);
             if (!self || k == 0) store(snapX + (M - k), xm);
@@ -527,137 +528,190 @@ k_pass2(FusedArgs a, FusedShape sh) {
             const cf we_k = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(k) * steps_n));
             cf xk, xm;
             real_split(zk, zm, W, we_k, k, xk, xm);
-            P[k] = (xk.x * xk.x + xk.y * xk.y) * inv_n;
+            if (k < lim) P[k] = (xk.x * xk.x + xk.y * xk.y) * inv_n;
             if constexpr (SNAP) store(snapX + k, xk);
         }
     }
 }
 
 // ---------------------------------------------------------------- harmonic --
-// Bins i < H < 2^24, so every index of the dyadic sums, (i k + 8) >> 4, is one 24-bit
-// multiply-add and a shift.  A thread owns the bins i, i + 256, i + 512, i + 768 of a
-// tile; bin i + 256 u reads index (i k + 8) >> 4 + 16 k u, so the four bins share one
-// set of index computations and differ only by constant offsets.
-constexpr int kHarmThreads = 256;
-constexpr int kPerThread = kTile / kHarmThreads;
+// Each lane owns 16 consecutive bins i = B + d (d < 16, B a multiple of 16).  The dyadic sum
+// of bin i reads P[(i k + 8) >> 4] for k = 1 .. 16, and (B + d) k + 8 >> 4 = B k / 16 +
+// ((d k + 8) >> 4), so for each k the lane reads one run of (15 k + 8 >> 4) + 1 consecutive
+// powers from B k / 16 (144 words for its 16 bins instead of 256 single-word gathers; the
+// runs of neighbouring lanes are adjacent, so a wave reads one stretch per k), and the bin
+// offsets within a run are compile-time constants.
+// A block is one wave: lane 0 covers the 16 bins before the tile and produces nothing (its
+// last 8 sums of each level feed lane 1), lanes 1 .. 63 own the tile's 1008 bins.  The
+// intervals of level h, [j 2^h - 2^(h-1), j 2^h + 2^(h-1)), start at most 8 bins before the
+// owning lane's first bin: each lane takes its left neighbour's last 8 sums of the level by
+// a wave shift and finds every maximum in registers.
+constexpr int kBinsPerLane = 16;
+constexpr int kTileBins = 63 * kBinsPerLane;  // bins owned per block
 
-// LDS slot of tile bin d (0 <= d < kTile + 8) in a row of sums: one pad word per 16, so
-// the interval reads of level h (lanes 2^h bins apart) spread over the banks.
-__device__ __forceinline__ int sum_at(int d) { return d + (d >> 4); }
-constexpr int kSumRow = kTile + 8 + (kTile + 8) / 16 + 1;
+constexpr int run_width(int k) { return ((15 * k + 8) >> 4) + 1; }
+
+template <int K>
+struct Run { float v[run_width(K)]; };
+
+template <int K>
+__device__ __forceinline__ Run<K> load_run(const float* __restrict__ P, int B) {
+    Run<K> r;
+    __builtin_memcpy(&r, P + (B / 16) * K, sizeof(r));  // B k / 16, exact: B is a multiple of 16
+    return r;
+}
+
+// The source of bin d in harmonic k's run.
+template <int K>
+__device__ __forceinline__ float src(const Run<K>& r, int d) { return r.v[(d * K + 8) >> 4]; }
+
+// v[d] of the lane's 8 predecessors (ext[0 .. 8), from the left neighbour) and own bins
+// (ext[8 .. 24)).
+__device__ __forceinline__ float shift_in(float x) {
+    // wave_shr:1 -- lane l receives lane l - 1's value (lane 0 keeps its own, unused)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, x),
+                                                                 __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, false));
+}
+
+// Level h (width 2^h) of the lane's fundamentals j = B / 2^h + m, m < 16 / 2^h: the value
+// of the reference's gather form (shaders/harmonic.comp:14-36): the best sum of the interval
+// within [W, H) if it passes the threshold, else the sum at the interval's first bin in
+// [W, H), 0 for an interval with none.  ext holds the sums of bins B - 8 .. B + 15, -inf
+// outside [W, H).  EXACT: every bin of the wave's intervals lies in [W, H).
+template <int LG, bool EXACT, bool SNAP>
+__device__ __forceinline__ void level_out(const float* ext, int B, int W, int H, int F, float thr, uint32_t* res,
+                                          uint32_t capacity, float* snap_row) {
+    constexpr int width = 1 << LG, half = width / 2, count = 16 / width;
+#pragma unroll
+    for (int m = 0; m < count; ++m) {
+        const int j = B / width + m;
+        if (j >= F) break;
+        const int lo = 8 + m * width - half;  // ext index of the interval's first bin
+        float best = ext[lo];
+#pragma unroll
+        for (int d = 1; d < width; ++d) best = fmaxf(best, ext[lo + d]);
+        float value;
+        if constexpr (EXACT) {
+            value = best > thr ? best : ext[lo];
+        } else {  // clip the interval to [W, H)
+            const int start = j * width - half;
+            const int begin = max(W, start), end = min(H, start + width);
+            float first = ext[lo];
+#pragma unroll
+            for (int d = 1; d < width; ++d)
+                if (start + d == begin) first = ext[lo + d];
+            value = begin >= end ? 0.0f : (best > thr ? best : first);
+        }
+        if constexpr (SNAP) snap_row[j] = value;
+        if (j >= W && value > thr) append_candidate(res, capacity, uint32_t(j), uint32_t(LG), value);
+    }
+}
+
+// One lane's 16 bins from B (lane 0: the 16 before the tile, which produce nothing).
+template <bool EXACT, bool SNAP>
+__device__ __forceinline__ void harmonic_lane(const float* __restrict__ P, const brp_params& p, int B, bool owner,
+                                              uint32_t* res, float* snap, uint32_t snap_f) {
+    const int W = int(p.window), H = int(p.harmonic), F = int(p.fundamental);
+    const int Bc = max(B, 0);  // run addresses (tile 0's lane 0 reads from bin 0 on)
+    float v[16], ext[24];
+    // ext = left neighbour's last 8 sums ++ own 16, -inf outside [W, H) (EXACT: none are).
+    auto extend = [&]() {
+#pragma unroll
+        for (int d = 0; d < 16; ++d) {
+            if constexpr (EXACT) ext[8 + d] = v[d];
+            else ext[8 + d] = (B + d >= W && B + d < H) ? v[d] : -INFINITY;
+        }
+#pragma unroll
+        for (int d = 0; d < 8; ++d) ext[d] = shift_in(ext[16 + d]);
+    };
+
+    // Level 0 and the start of the sums: k = 16 (the bins themselves) and k = 8.
+    {
+        const Run<16> r16 = load_run<16>(P, Bc);
+        const Run<8> r8 = load_run<8>(P, Bc);
+        if (owner && B < F) {
+            const float thr0 = p.threshold[0];
+#pragma unroll
+            for (int d = 0; d < 16; ++d) {
+                const int i = B + d;
+                if (i < F) {
+                    if constexpr (SNAP) snap[i] = r16.v[d];
+                    if (i >= W && r16.v[d] > thr0) append_candidate(res, p.capacity, uint32_t(i), 0, r16.v[d]);
+                }
+            }
+        }
+        // Historical order (shaders/hs_sum.glsl:2-10), at(k) = P[(i k + 8) >> 4].
+#pragma unroll
+        for (int d = 0; d < 16; ++d) v[d] = src(r16, d) + src(r8, d);
+    }
+    extend();
+    if (owner && B < 2 * F)
+        level_out<1, EXACT, SNAP>(ext, B, W, H, F, p.threshold[1], res, p.capacity, snap + 1 * snap_f);
+    {
+        const Run<12> r12 = load_run<12>(P, Bc);
+        const Run<4> r4 = load_run<4>(P, Bc);
+#pragma unroll
+        for (int d = 0; d < 16; ++d) v[d] += src(r12, d) + src(r4, d);
+    }
+    extend();
+    if (owner && B < 4 * F)
+        level_out<2, EXACT, SNAP>(ext, B, W, H, F, p.threshold[2], res, p.capacity, snap + 2 * snap_f);
+    {
+        const Run<14> r14 = load_run<14>(P, Bc);
+        const Run<10> r10 = load_run<10>(P, Bc);
+        const Run<6> r6 = load_run<6>(P, Bc);
+        const Run<2> r2 = load_run<2>(P, Bc);
+#pragma unroll
+        for (int d = 0; d < 16; ++d) v[d] += src(r14, d) + src(r10, d) + src(r6, d) + src(r2, d);
+    }
+    extend();
+    if (owner && B < 8 * F)
+        level_out<3, EXACT, SNAP>(ext, B, W, H, F, p.threshold[3], res, p.capacity, snap + 3 * snap_f);
+    {
+        float in[16];
+        {
+            const Run<15> r15 = load_run<15>(P, Bc);
+            const Run<13> r13 = load_run<13>(P, Bc);
+#pragma unroll
+            for (int d = 0; d < 16; ++d) in[d] = src(r15, d) + src(r13, d);
+        }
+        {
+            const Run<11> r11 = load_run<11>(P, Bc);
+            const Run<9> r9 = load_run<9>(P, Bc);
+#pragma unroll
+            for (int d = 0; d < 16; ++d) in[d] = (in[d] + src(r11, d)) + src(r9, d);
+        }
+        {
+            const Run<7> r7 = load_run<7>(P, Bc);
+            const Run<5> r5 = load_run<5>(P, Bc);
+            const Run<3> r3 = load_run<3>(P, Bc);
+            const Run<1> r1 = load_run<1>(P, Bc);
+#pragma unroll
+            for (int d = 0; d < 16; ++d) v[d] += (((in[d] + src(r7, d)) + src(r5, d)) + src(r3, d)) + src(r1, d);
+        }
+    }
+    extend();
+    if (owner && B < 16 * F)
+        level_out<4, EXACT, SNAP>(ext, B, W, H, F, p.threshold[4], res, p.capacity, snap + 4 * snap_f);
+}
 
 template <bool SNAP>
-__global__ __launch_bounds__(kHarmThreads) void k_harmonic(FusedArgs a, FusedShape sh, uint32_t tiles) {
-    __shared__ float sums[4][kSumRow];
-    const int t = threadIdx.x;
+__global__ __launch_bounds__(64) void k_harmonic(FusedArgs a, FusedShape sh, uint32_t tiles) {
+    const int lane = threadIdx.x;
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t g = item / tiles, tile = item % tiles;
     const uint32_t slot = a.first + g;
     const brp_params& p = a.params[slot];
     const float* __restrict__ P = a.P + size_t(g) * sh.m;
-    const int i0 = int(tile) * kTile;
-    const int W = int(p.window), H = int(p.harmonic), F = int(p.fundamental);
-    const float thr0 = p.threshold[0];
     uint32_t* res = a.results + size_t(slot) * a.slot_words;
     float* snap = SNAP ? a.snap_summed + size_t(slot) * 5 * a.snap_f : nullptr;
-
-    // Phase A: dyadic sums of the tile's bins i0 .. i0 + kTile - 1 (LDS slots 8 ..) and
-    // of the 8 bins before it (slots 0 .. 7), -inf outside [W, H); level-0 candidates.
-    // Every source value of the thread's bins is read first, unconditionally (the power
-    // buffer has room past the last bin a tile can name), so all loads are in flight
-    // together; bins outside [W, H) are masked afterwards.
-    {
-        const uint32_t ib = uint32_t(i0 + t);
-        uint32_t idx[16];
-#pragma unroll
-        for (int k = 1; k <= 16; ++k) idx[k - 1] = (__umul24(ib, uint32_t(k)) + 8u) >> 4;
-        float val[kPerThread][16];
-#pragma unroll
-        for (int u = 0; u < kPerThread; ++u)
-#pragma unroll
-            for (int k = 1; k <= 16; ++k) val[u][k - 1] = P[idx[k - 1] + uint32_t(16 * k * u)];
-        const bool halo = t < 8;
-        const int ih = i0 - 8 + t;
-        float hval[16];
-        if (halo) {
-            const uint32_t ic = uint32_t(max(ih, 0));
-#pragma unroll
-            for (int k = 1; k <= 16; ++k) hval[k - 1] = P[(__umul24(ic, uint32_t(k)) + 8u) >> 4];
-        }
-        // Historical order (shaders/hs_sum.glsl:2-10); at(k) = P[(i k + 8) >> 4].
-        auto dyadic = [](const float* at, float s[4]) {
-            float v = at[15];
-            v += at[7]; s[0] = v;
-            v += at[11] + at[3]; s[1] = v;
-            v += at[13] + at[9] + at[5] + at[1]; s[2] = v;
-            v += at[14] + at[12] + at[10] + at[8] + at[6] + at[4] + at[2] + at[0]; s[3] = v;
-        };
-#pragma unroll
-        for (int u = 0; u < kPerThread; ++u) {
-            const int i = i0 + t + kHarmThreads * u;
-            float s[4];
-            dyadic(val[u], s);
-            const bool live = i >= W && i < H;
-            const int d = sum_at(8 + t + kHarmThreads * u);
-#pragma unroll
-            for (int h = 0; h < 4; ++h) sums[h][d] = live ? s[h] : -INFINITY;
-            if (i < F) {
-                const float v0 = val[u][15];  // P[i]
-                if constexpr (SNAP) snap[i] = v0;
-                if (i >= W && v0 > thr0) append_candidate(res, p.capacity, uint32_t(i), 0, v0);
-            }
-        }
-        if (halo) {
-            float s[4];
-            dyadic(hval, s);
-            const bool live = ih >= W && ih < H;
-            const int d = sum_at(t);
-#pragma unroll
-            for (int h = 0; h < 4; ++h) sums[h][d] = live ? s[h] : -INFINITY;
-        }
-    }
-    __syncthreads();
-    // Phase B: level h, fundamental j owns bins [j 2^h - 2^(h-1), j 2^h + 2^(h-1)) ∩ [W, H);
-    // the tile owns the j with j 2^h in [i0, i0 + kTile), whose bins lie in its LDS rows
-    // (slots gi 2^h + 8 - 2^(h-1) .. for gi = j - j0).  Sums outside [W, H) are -inf, so the
-    // max over the whole interval is the max over its part in [W, H): all 2^h reads are
-    // issued together and reduced as a tree.  The value is the best sum if it passes the
-    // threshold, else the sum at the interval's first bin in [W, H), and 0 for an interval
-    // with none.  Levels 3 and 4 (128 and 64 fundamentals) go to different waves.
-#pragma unroll
-    for (int h = 1; h <= 4; ++h) {
-        constexpr int kLane[5] = {0, 0, 0, 128, 0};  // first thread of the level's groups
-        const int width = 1 << h;
-        const int groups = kTile >> h;
-        const int j0 = i0 >> h;
-        const float thr = p.threshold[h];
-        const float* row = sums[h - 1];
-#pragma unroll
-        for (int pass = 0; pass < (groups + kHarmThreads - 1) / kHarmThreads; ++pass) {
-            const int gi = t - kLane[h] + pass * kHarmThreads;
-            const int j = j0 + gi;
-            if (gi < 0 || gi >= groups || j >= F) continue;
-            const int start = j * width - width / 2;  // first bin of the interval
-            const int lo = gi * width + 8 - width / 2;
-            float v[1 << 4];
-#pragma unroll
-            for (int d = 0; d < width; ++d) v[d] = row[sum_at(lo + d)];
-            const float at_start = v[0];
-#pragma unroll
-            for (int span = 1; span < width; span *= 2)
-#pragma unroll
-                for (int d = 0; d < width; d += 2 * span) v[d] = fmaxf(v[d], v[d + span]);
-            const int begin = max(W, start);
-            const int end = min(H, start + width);
-            float value = 0.0f;
-            if (begin < end) {
-                const float first = begin == start ? at_start : row[sum_at(lo + begin - start)];
-                value = v[0] > thr ? v[0] : first;
-            }
-            if constexpr (SNAP) snap[size_t(h) * a.snap_f + j] = value;
-            if (j >= W && value > thr) append_candidate(res, p.capacity, uint32_t(j), uint32_t(h), value);
-        }
-    }
+    const int i0 = int(tile) * kTileBins;
+    const int B = i0 + (lane - 1) * kBinsPerLane;
+    // Every bin the wave's intervals touch (from 8 before the tile) in [W, H): no clipping.
+    if (i0 - 8 >= int(p.window) && i0 + kTileBins <= int(p.harmonic))
+        harmonic_lane<true, SNAP>(P, p, B, lane > 0, res, snap, a.snap_f);
+    else
+        harmonic_lane<false, SNAP>(P, p, B, lane > 0, res, snap, a.snap_f);
 }
 
 __global__ void k_pack_results(const uint32_t* __restrict__ results, uint32_t slot_words, uint32_t nslots,
@@ -890,9 +944,16 @@ void fused_pass2(hipStream_t st, const FusedShape& s, const FusedArgs& a) {
     with_pass2(s.r, s.s2, s.t2, [&]<int R, int S, int T>() { launch2<R, S, T>(st, s, a); });
 }
 
-void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t tiles) {
-    if (a.snap_summed) k_harmonic<true><<<tiles * a.count, kHarmThreads, 0, st>>>(a, s, tiles);
-    else k_harmonic<false><<<tiles * a.count, kHarmThreads, 0, st>>>(a, s, tiles);
+uint32_t harmonic_power_bins(uint32_t reach) {
+    // Lane 63 of the last tile reads P[B .. B + 16) with B = tiles kTileBins - 16; the runs of
+    // k < 16 end lower.
+    return (reach + kTileBins - 1) / kTileBins * kTileBins;
+}
+
+void fused_harmonic(hipStream_t st, const FusedShape& s, const FusedArgs& a, uint32_t reach) {
+    const uint32_t tiles = (reach + kTileBins - 1) / kTileBins;
+    if (a.snap_summed) k_harmonic<true><<<tiles * a.count, 64, 0, st>>>(a, s, tiles);
+    else k_harmonic<false><<<tiles * a.count, 64, 0, st>>>(a, s, tiles);
 }
 
 void search_steps(hipStream_t st, brp_params* params, uint32_t count, const float2* lut, uint32_t* error) {

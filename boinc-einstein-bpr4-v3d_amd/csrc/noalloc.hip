@@ -209,9 +209,10 @@ extern "C" int brp_hip_search_noalloc(const brp_params* templates, size_t count,
             reach = std::max<uint64_t>(reach, std::min<uint64_t>(16ull * p.fundamental, uint64_t(p.harmonic) + 8));
         }
         a.nonneg = nonneg ? 1u : 0u;
+        a.power_bins = brp_engine::harmonic_power_bins(uint32_t(reach));
         brp_engine::fused_pass1(st, sh, a);
         brp_engine::fused_pass2(st, sh, a);
-        brp_engine::fused_harmonic(st, sh, a, uint32_t((reach + 1023) / 1024));
+        brp_engine::fused_harmonic(st, sh, a, uint32_t(reach));
         hip_ok(hipGetLastError(), "no-allocation search launch");
         return BRP_OK;
     } catch (const Error& e) {
