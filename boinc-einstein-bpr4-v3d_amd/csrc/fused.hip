@@ -561,6 +561,75 @@ __device__ __forceinline__ Run<K> load_run(const float* __restrict__ P, int B) {
     return r;
 }
 
+// Runs of k >= 5 go through LDS.  Loaded straight from P, a wave instruction of lane-owned
+// runs touches lanes k words apart (about 2 k cache lines per instruction, one tag lookup
+// each), and the L1 tag rate bounds the kernel.  Instead the wave loads the stretch of P its
+// 64 runs cover, [s k, s k + 63 k + width), as contiguous 16-byte pieces (lane m takes words
+// 4 m + 256 j), writes it to its LDS buffer and reads each lane's run from there.  A lane's
+// run starts lb k words in (lb: the lane's 16-bin block from the wave's first); for k = 8, 12
+// and 16 one padding word follows every k words, so that the 32 lanes of an LDS cycle read
+// with an odd stride (k + 1) and hit distinct banks; the other k are odd (no conflict) or
+// 2 mod 4 (two-way).  A wave reads and writes only its own buffer, and a wave executes its LDS
+// instructions in order, so no barrier separates the phases.
+constexpr bool staged(int k) { return k >= 5; }
+constexpr bool padded(int k) { return k == 8 || k == 12 || k == 16; }
+constexpr int stretch_len(int k) { return 63 * k + run_width(k); }
+constexpr int pieces(int k) { return (stretch_len(k) + 255) / 256; }
+constexpr int stage_words(int k) { return padded(k) ? 256 * pieces(k) + 256 * pieces(k) / k : 256 * pieces(k); }
+constexpr int kStageWords = 1088;  // the largest stage_words(k), k = 5 .. 16
+constexpr bool stage_fits() {
+    for (int k = 5; k <= 16; ++k)
+        if (stage_words(k) > kStageWords) return false;
+    return true;
+}
+static_assert(stage_fits(), "stage buffer");
+
+template <int K>
+struct Pieces { float4 c[pieces(K)]; };
+
+// The wave's stretch of harmonic K from P + s K.  Lanes whose piece lies past the stretch
+// load the stretch's last 4 words instead (no branch, no extra cache line); their pieces
+// land past the stretch in LDS, where no run reads.
+template <int K>
+__device__ __forceinline__ Pieces<K> fetch_stretch(const float* __restrict__ P, int s, int m) {
+    Pieces<K> pc;
+    const float* src = P + s * K;
+#pragma unroll
+    for (int j = 0; j < pieces(K); ++j)
+        __builtin_memcpy(&pc.c[j], src + min(4 * m + 256 * j, stretch_len(K) - 4), 16);
+    return pc;
+}
+
+// LDS word of stretch word e.
+template <int K>
+__device__ __forceinline__ int stage_at(int e) { return padded(K) ? e + e / K : e; }
+
+template <int K>
+__device__ __forceinline__ Run<K> stage_run(float* __restrict__ buf, const Pieces<K>& pc, int m, int lb) {
+    asm volatile("" ::: "memory");  // the previous stretch's reads stay before these writes
+#pragma unroll
+    for (int j = 0; j < pieces(K); ++j) {
+        const int e = 4 * m + 256 * j;
+        if constexpr (padded(K)) {
+            // The 4 words of a piece share one padding offset (e is a multiple of 4 and
+            // e mod k + 3 < k for k = 8, 12, 16).
+            float* d = buf + stage_at<K>(e);
+            d[0] = pc.c[j].x;
+            d[1] = pc.c[j].y;
+            d[2] = pc.c[j].z;
+            d[3] = pc.c[j].w;
+        } else {
+            *reinterpret_cast<float4*>(buf + e) = pc.c[j];
+        }
+    }
+    asm volatile("" ::: "memory");
+    Run<K> r;
+    const float* src = buf + (padded(K) ? lb * (K + 1) : lb * K);
+#pragma unroll
+    for (int d = 0; d < run_width(K); ++d) r.v[d] = src[padded(K) && d >= K ? d + 1 : d];
+    return r;
+}
+
 // The source of bin d in harmonic k's run.
 template <int K>
 __device__ __forceinline__ float src(const Run<K>& r, int d) { return r.v[(d * K + 8) >> 4]; }
@@ -607,11 +676,20 @@ __device__ __forceinline__ void level_out(const float* ext, int B, int W, int H,
     }
 }
 
-// One lane's 16 bins from B (lane 0: the 16 before the tile, which produce nothing).
+// One lane's 16 bins from B (lane 0: the 16 before the tile, which produce nothing).  s: the
+// wave's first 16-bin block (stretch start / k), lb: the lane's block from s, m: the lane.
+// The stretches are fetched two ahead of their use, in the order the sums need them.
 template <bool EXACT, bool SNAP>
 __device__ __forceinline__ void harmonic_lane(const float* __restrict__ P, const brp_params& p, int B, bool owner,
-                                              uint32_t* res, float* snap, uint32_t snap_f) {
+                                              uint32_t* res, float* snap, uint32_t snap_f, float* __restrict__ buf,
+                                              int s, int lb, int m) {
+    // The template's fields in registers: the LDS phases below are compiler memory barriers,
+    // across which fields read from memory would be read again.
     const int W = int(p.window), H = int(p.harmonic), F = int(p.fundamental);
+    const uint32_t cap = p.capacity;
+    float thr[5];
+#pragma unroll
+    for (int h = 0; h < 5; ++h) thr[h] = p.threshold[h];
     const int Bc = max(B, 0);  // run addresses (tile 0's lane 0 reads from bin 0 on)
     float v[16], ext[24];
     // ext = left neighbour's last 8 sums ++ own 16, -inf outside [W, H) (EXACT: none are).
@@ -625,74 +703,78 @@ __device__ __forceinline__ void harmonic_lane(const float* __restrict__ P, const
         for (int d = 0; d < 8; ++d) ext[d] = shift_in(ext[16 + d]);
     };
 
+    const Run<4> r4 = load_run<4>(P, Bc);
+    const Run<2> r2 = load_run<2>(P, Bc);
+    const Run<3> r3 = load_run<3>(P, Bc);
+    const Run<1> r1 = load_run<1>(P, Bc);
+    const Pieces<16> p16 = fetch_stretch<16>(P, s, m);
+    const Pieces<8> p8 = fetch_stretch<8>(P, s, m);
+
     // Level 0 and the start of the sums: k = 16 (the bins themselves) and k = 8.
     {
-        const Run<16> r16 = load_run<16>(P, Bc);
-        const Run<8> r8 = load_run<8>(P, Bc);
+        const Run<16> r16 = stage_run<16>(buf, p16, m, lb);
         if (owner && B < F) {
-            const float thr0 = p.threshold[0];
 #pragma unroll
             for (int d = 0; d < 16; ++d) {
                 const int i = B + d;
                 if (i < F) {
                     if constexpr (SNAP) snap[i] = r16.v[d];
-                    if (i >= W && r16.v[d] > thr0) append_candidate(res, p.capacity, uint32_t(i), 0, r16.v[d]);
+                    if (i >= W && r16.v[d] > thr[0]) append_candidate(res, cap, uint32_t(i), 0, r16.v[d]);
                 }
             }
         }
+        const Pieces<12> p12 = fetch_stretch<12>(P, s, m);
+        const Run<8> r8 = stage_run<8>(buf, p8, m, lb);
+        const Pieces<14> p14 = fetch_stretch<14>(P, s, m);
         // Historical order (shaders/hs_sum.glsl:2-10), at(k) = P[(i k + 8) >> 4].
 #pragma unroll
         for (int d = 0; d < 16; ++d) v[d] = src(r16, d) + src(r8, d);
-    }
-    extend();
-    if (owner && B < 2 * F)
-        level_out<1, EXACT, SNAP>(ext, B, W, H, F, p.threshold[1], res, p.capacity, snap + 1 * snap_f);
-    {
-        const Run<12> r12 = load_run<12>(P, Bc);
-        const Run<4> r4 = load_run<4>(P, Bc);
+        extend();
+        if (owner && B < 2 * F)
+            level_out<1, EXACT, SNAP>(ext, B, W, H, F, thr[1], res, cap, snap + 1 * snap_f);
+        {
+            const Run<12> r12 = stage_run<12>(buf, p12, m, lb);
 #pragma unroll
-        for (int d = 0; d < 16; ++d) v[d] += src(r12, d) + src(r4, d);
-    }
-    extend();
-    if (owner && B < 4 * F)
-        level_out<2, EXACT, SNAP>(ext, B, W, H, F, p.threshold[2], res, p.capacity, snap + 2 * snap_f);
-    {
-        const Run<14> r14 = load_run<14>(P, Bc);
-        const Run<10> r10 = load_run<10>(P, Bc);
-        const Run<6> r6 = load_run<6>(P, Bc);
-        const Run<2> r2 = load_run<2>(P, Bc);
+            for (int d = 0; d < 16; ++d) v[d] += src(r12, d) + src(r4, d);
+        }
+        const Pieces<10> p10 = fetch_stretch<10>(P, s, m);
+        extend();
+        if (owner && B < 4 * F)
+            level_out<2, EXACT, SNAP>(ext, B, W, H, F, thr[2], res, cap, snap + 2 * snap_f);
+        const Run<14> r14 = stage_run<14>(buf, p14, m, lb);
+        const Pieces<6> p6 = fetch_stretch<6>(P, s, m);
+        const Run<10> r10 = stage_run<10>(buf, p10, m, lb);
+        const Pieces<15> p15 = fetch_stretch<15>(P, s, m);
+        const Run<6> r6 = stage_run<6>(buf, p6, m, lb);
+        const Pieces<13> p13 = fetch_stretch<13>(P, s, m);
 #pragma unroll
         for (int d = 0; d < 16; ++d) v[d] += src(r14, d) + src(r10, d) + src(r6, d) + src(r2, d);
-    }
-    extend();
-    if (owner && B < 8 * F)
-        level_out<3, EXACT, SNAP>(ext, B, W, H, F, p.threshold[3], res, p.capacity, snap + 3 * snap_f);
-    {
+        extend();
+        if (owner && B < 8 * F)
+            level_out<3, EXACT, SNAP>(ext, B, W, H, F, thr[3], res, cap, snap + 3 * snap_f);
         float in[16];
         {
-            const Run<15> r15 = load_run<15>(P, Bc);
-            const Run<13> r13 = load_run<13>(P, Bc);
+            const Run<15> r15 = stage_run<15>(buf, p15, m, lb);
+            const Pieces<11> p11 = fetch_stretch<11>(P, s, m);
+            const Run<13> r13 = stage_run<13>(buf, p13, m, lb);
+            const Pieces<9> p9 = fetch_stretch<9>(P, s, m);
 #pragma unroll
             for (int d = 0; d < 16; ++d) in[d] = src(r15, d) + src(r13, d);
-        }
-        {
-            const Run<11> r11 = load_run<11>(P, Bc);
-            const Run<9> r9 = load_run<9>(P, Bc);
+            const Run<11> r11 = stage_run<11>(buf, p11, m, lb);
+            const Pieces<7> p7 = fetch_stretch<7>(P, s, m);
+            const Run<9> r9 = stage_run<9>(buf, p9, m, lb);
+            const Pieces<5> p5 = fetch_stretch<5>(P, s, m);
 #pragma unroll
             for (int d = 0; d < 16; ++d) in[d] = (in[d] + src(r11, d)) + src(r9, d);
-        }
-        {
-            const Run<7> r7 = load_run<7>(P, Bc);
-            const Run<5> r5 = load_run<5>(P, Bc);
-            const Run<3> r3 = load_run<3>(P, Bc);
-            const Run<1> r1 = load_run<1>(P, Bc);
+            const Run<7> r7 = stage_run<7>(buf, p7, m, lb);
+            const Run<5> r5 = stage_run<5>(buf, p5, m, lb);
 #pragma unroll
             for (int d = 0; d < 16; ++d) v[d] += (((in[d] + src(r7, d)) + src(r5, d)) + src(r3, d)) + src(r1, d);
         }
     }
     extend();
     if (owner && B < 16 * F)
-        level_out<4, EXACT, SNAP>(ext, B, W, H, F, p.threshold[4], res, p.capacity, snap + 4 * snap_f);
+        level_out<4, EXACT, SNAP>(ext, B, W, H, F, thr[4], res, cap, snap + 4 * snap_f);
 }
 
 template <bool SNAP>
@@ -705,13 +787,16 @@ __global__ __launch_bounds__(64) void k_harmonic(FusedArgs a, FusedShape sh, uin
     const float* __restrict__ P = a.P + size_t(g) * sh.m;
     uint32_t* res = a.results + size_t(slot) * a.slot_words;
     float* snap = SNAP ? a.snap_summed + size_t(slot) * 5 * a.snap_f : nullptr;
+    __shared__ float buf[kStageWords];
     const int i0 = int(tile) * kTileBins;
     const int B = i0 + (lane - 1) * kBinsPerLane;
+    const int s = max(i0 / kBinsPerLane - 1, 0);
+    const int lb = max(B, 0) / kBinsPerLane - s;
     // Every bin the wave's intervals touch (from 8 before the tile) in [W, H): no clipping.
     if (i0 - 8 >= int(p.window) && i0 + kTileBins <= int(p.harmonic))
-        harmonic_lane<true, SNAP>(P, p, B, lane > 0, res, snap, a.snap_f);
+        harmonic_lane<true, SNAP>(P, p, B, lane > 0, res, snap, a.snap_f, buf, s, lb, lane);
     else
-        harmonic_lane<false, SNAP>(P, p, B, lane > 0, res, snap, a.snap_f);
+        harmonic_lane<false, SNAP>(P, p, B, lane > 0, res, snap, a.snap_f, buf, s, lb, lane);
 }
 
 __global__ void k_pack_results(const uint32_t* __restrict__ results, uint32_t slot_words, uint32_t nslots,
