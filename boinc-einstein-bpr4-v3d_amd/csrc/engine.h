@@ -44,6 +44,7 @@ struct FusedShape {
     uint32_t t1 = 0, t2 = 0;              // threads per pass-1 / pass-2 block
     uint32_t blocks1 = 0, blocks2 = 0;    // blocks per template
     uint32_t mirror = 0;                  // exchange rows with each column beside its mirror
+    uint32_t n_shift = 0, n_odd = 0;      // N = n_odd 2^n_shift, n_odd 1 or 3
 };
 
 struct FusedArgs {

@@ -183,8 +183,9 @@ __device__ __forceinline__ uint32_t exchange_pos(uint32_t k2, uint32_t C) {
 // b = clamp(j0, 0, U - 1) serves both (the observation buffer holds zero guard samples
 // after U - 1); any other pair takes a second load.  Samples outside the observation
 // are masked to zero afterwards and flagged.
-// NN: phase >= 0 for every sample; FM: flag 2 arithmetic; F64: FP64 orbital phase.
-template <int C, int S, int T, bool SNAP, bool F64, bool NN, bool FM>
+// NN: phase >= 0 for every sample; FM: flag 2 arithmetic; F64: FP64 orbital phase; ALL:
+// every sample of the block lies below steps (all blocks but the last few of a template).
+template <int C, int S, int T, bool SNAP, bool F64, bool NN, bool FM, bool ALL>
 __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, const FusedArgs& a,
                                                const FusedShape& sh, const brp_params& p,
                                                const float4* __restrict__ lut4, uint32_t r0, uint32_t slot,
@@ -253,7 +254,7 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
             for (int h = 0; h < 2; ++h) {
                 const uint32_t i = 2 * m + h;
                 const int jj = src[2 * e + h];
-                const bool in_series = i < steps;
+                const bool in_series = ALL || i < steps;
                 const bool inside = uint32_t(jj) < U;
                 bad |= in_series && !inside;
                 pair[h] = (in_series && inside) ? y[2 * e + h] : 0.0f;
@@ -315,7 +316,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 
     double acc = 0.0;
     bool bad = false;
     cf v[E];
-    resample_pairs<C, S, T, SNAP, F64, NN, FM>(v, acc, bad, a, sh, p, lut4, r0, slot, t);
+    // The block's last sample: m = r0 + S - 1 + R (C - 1), i = 2 m + 1.
+    if (2u * (r0 + S - 1 + sh.r * (C - 1)) + 1u < p.steps)
+        resample_pairs<C, S, T, SNAP, F64, NN, FM, true>(v, acc, bad, a, sh, p, lut4, r0, slot, t);
+    else
+        resample_pairs<C, S, T, SNAP, F64, NN, FM, false>(v, acc, bad, a, sh, p, lut4, r0, slot, t);
     fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
 
     // Y[r][k2] = v * W_M^{r k2} for k2 = j + q C/RL: W_M^{r j} times
@@ -365,6 +370,17 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 
     }
 }
 
+// (a b) mod N for a, b < N, N = odd 2^shift with odd 1 or 3: the low `shift` bits of a b stay
+// and the rest reduces mod odd (a b < N^2, so (a b) >> shift < 3 N fits in 32 bits; hi / 3 =
+// umulhi(hi, 0xAAAAAAAB) / 2 exactly for every 32-bit hi).
+__device__ __forceinline__ uint32_t mulmod_n(uint32_t a, uint32_t b, const FusedShape& sh) {
+    const uint64_t x = uint64_t(a) * b;
+    const uint32_t lo = uint32_t(x) & ((1u << sh.n_shift) - 1u);
+    uint32_t hi = uint32_t(x >> sh.n_shift);
+    hi = sh.n_odd == 3u ? hi - 3u * (__umulhi(hi, 0xAAAAAAABu) >> 1) : 0u;
+    return (hi << sh.n_shift) | lo;
+}
+
 // ------------------------------------------------------------------ pass 2 --
 // SNAP: also write the spectrum X (validation runs).
 // MF: the split-FP16 MFMA variant of the radix-16 stages (fewer waves per SIMD: the
@@ -401,7 +417,8 @@ k_pass2(FusedArgs a, FusedShape sh) {
     const double* partial = a.partial + size_t(g) * sh.blocks1;
     double part = uint32_t(t) < sh.blocks1 ? partial[t] : 0.0;
 
-    const float2* Y = a.Y + size_t(g) * M;
+    // Column reads as 32-bit offsets from the slot's base (M complex < 2^32 bytes).
+    const char* Y = reinterpret_cast<const char*>(a.Y + size_t(g) * M);
     cf v[E];
 #pragma unroll
     for (int u = 0; u < E / R0; ++u) {
@@ -409,8 +426,12 @@ k_pass2(FusedArgs a, FusedShape sh) {
         const int s = b % SS, j = b / SS;
         const uint32_t col = column(s);
         const uint32_t pos = sh.mirror ? exchange_pos(col, C) : col;
+        const uint32_t at = (uint32_t(j) * C + pos) * uint32_t(sizeof(float2));
 #pragma unroll
-        for (int q = 0; q < R0; ++q) v[u * R0 + q] = load(Y + size_t(j + q * (R / R0)) * C + pos);
+        for (int q = 0; q < R0; ++q) {
+            const uint32_t byte = at + uint32_t(q * (R / R0)) * C * uint32_t(sizeof(float2));
+            v[u * R0 + q] = load(reinterpret_cast<const float2*>(Y + byte));
+        }
     }
 
     // Pad mean from the pass-1 partial sums, in a fixed order (bit-reproducible), while
@@ -457,13 +478,10 @@ k_pass2(FusedArgs a, FusedShape sh) {
     // constant D = (T/S) C steps mod N.  W_N^{e} starts from one table lookup and is then
     // multiplied by W_N^D per pair (the correction term tolerates the few-ulp drift).
     static_assert(T % S == 0, "pair mapping");
-    auto mod_n = [&](uint64_t x) -> uint32_t {
-        return uint32_t(x % uint64_t(N));
-    };
-    const uint32_t steps_n = steps % N;
+    const uint32_t steps_n = steps < N ? steps : steps - N;  // steps <= N
     const uint32_t k_first = uint32_t(t / S) * C + base + uint32_t(t % S);
-    cf we = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(k_first) * steps_n));
-    const cf wstep = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(T / S) * C % N * steps_n));
+    cf we = twiddle2(a.twn_lo, a.twn_hi, mulmod_n(k_first, steps_n, sh));
+    const cf wstep = twiddle2(a.twn_lo, a.twn_hi, mulmod_n(uint32_t(T / S) * C, steps_n, sh));
     // Every W_N^k of the thread's pairs first, so their table reads are in flight together.
     constexpr int PAIRS = R * S / T;
     cf Wk[PAIRS];
@@ -525,7 +543,7 @@ k_pass2(FusedArgs a, FusedShape sh) {
             const cf zk = load(lds + lds_at<SS>(int(k1), S));
             const cf zm = load(lds + lds_at<SS>(int(R - 1 - k1), S));
             const cf W = twiddle2(a.twn_lo, a.twn_hi, k);
-            const cf we_k = twiddle2(a.twn_lo, a.twn_hi, mod_n(uint64_t(k) * steps_n));
+            const cf we_k = twiddle2(a.twn_lo, a.twn_hi, mulmod_n(k, steps_n, sh));
             cf xk, xm;
             real_split(zk, zm, W, we_k, k, xk, xm);
             if (k < lim) P[k] = (xk.x * xk.x + xk.y * xk.y) * inv_n;
@@ -995,6 +1013,9 @@ bool shape_of(const ShapeRow& row, uint32_t n, FusedShape* out) {
     s.blocks1 = row.r / row.s1;
     s.blocks2 = row.c / (2 * row.s2);
     s.mirror = row.s2 <= 4;
+    s.n_shift = uint32_t(__builtin_ctz(n));
+    s.n_odd = n >> s.n_shift;
+    if (s.n_odd != 1 && s.n_odd != 3) return false;
     *out = s;
     return true;
 }
