@@ -347,9 +347,9 @@ def main() -> int:
     # --------------------------------------------------------------- roofline --
     B = workloads.algorithmic_bytes(n, U, H)
     # Per-kernel algorithmic bytes per template: pass 1 reads the series and writes the
-    # N/2-point exchange; pass 2 reads it back and writes the N/2 power bins; the
-    # harmonic kernel reads the power below H.
-    kb = {"pass1_resample_fft": 4 * U + 4 * n, "pass2_fft_power": 4 * n + 2 * n, "harmonic": 4 * H}
+    # N/2-point exchange; pass 2 reads it back and writes the power below H (the bins the
+    # harmonic sums read); the harmonic kernel reads that power.
+    kb = {"pass1_resample_fft": 4 * U + 4 * n, "pass2_fft_power": 4 * n + 4 * H, "harmonic": 4 * H}
     kms = {"pass1_resample_fft": km[0], "pass2_fft_power": km[1], "harmonic": km[3]}
     dom = max(kms, key=lambda k: kms[k])
     per_launch_ms = kms[dom] / max(launches, 1)
