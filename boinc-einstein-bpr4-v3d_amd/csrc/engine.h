@@ -51,8 +51,8 @@ struct FusedArgs {
     const brp_params* params = nullptr;  // per template slot (device)
     const float* x = nullptr;
     const float2* lut = nullptr;
-    const float2* tw_c = nullptr;    // W_C^t, t < C
-    const float2* tw_r = nullptr;    // W_R^t, t < R
+    const float2* tw_c = nullptr;    // pass-1 per-stage twiddle table (fft_stage_table(C, ...))
+    const float2* tw_r = nullptr;    // pass-2 per-stage twiddle table (fft_stage_table(R, ...))
     const float2* twm_lo = nullptr;  // W_M^t, t < 2048
     const float2* twm_hi = nullptr;  // W_M^{2048 q}
     const float2* tw_rq = nullptr;   // W_{16 R}^t, t < 16 R
@@ -87,6 +87,24 @@ bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShap
                      uint32_t t2 = 0);
 // Whether the shape's pass-2 plan starts with radix-16 stages (the MFMA variant's stages).
 bool fused_mfma_capable(const FusedShape& s);
+// The per-stage twiddle table of a fused FFT pass (length len, `elements` per thread): for
+// each stage i >= 1 of the radix plan (radix[i], ns[i] = product of the earlier radices) a
+// block at offset[i] whose entry (q - 1) ns[i] + k, q in [1, radix[i]), k < ns[i], holds
+// W_len^{k q len / (ns[i] radix[i])}; count entries in all.
+struct StageTable {
+    uint32_t len = 0, count = 0, stages = 0;
+    uint32_t offset[4] = {0, 0, 0, 0}, ns[4] = {0, 0, 0, 0}, radix[4] = {0, 0, 0, 0};
+    // The exponent of entry e (e < count).
+    __host__ __device__ uint64_t exponent(uint32_t e) const {
+        uint32_t i = 0;
+        while (i + 1 < stages && e >= offset[i + 1]) ++i;
+        const uint32_t within = e - offset[i], q = within / ns[i] + 1, k = within % ns[i];
+        return uint64_t(k) * q * (len / (ns[i] * radix[i]));
+    }
+};
+StageTable fft_stage_table(uint32_t len, uint32_t elements);
+uint32_t fused_pass1_elements(const FusedShape& s);  // C S1 / T1
+uint32_t fused_pass2_elements(const FusedShape& s);  // 2 R S2 / T2
 // Every compiled shape of series length n, the default first.
 std::vector<FusedShape> fused_shapes(uint32_t n);
 // One launch each covers all `count` templates of the group.

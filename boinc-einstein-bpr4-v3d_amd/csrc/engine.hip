@@ -62,6 +62,16 @@ std::vector<float2> twiddles(uint64_t len, uint64_t count, uint64_t stride = 1) 
     return w;
 }
 
+// The per-stage twiddle table of a fused FFT pass, each entry rounded as twiddles() rounds.
+std::vector<float2> stage_twiddles(const brp_engine::StageTable& tb) {
+    std::vector<float2> w(tb.count);
+    for (uint32_t e = 0; e < tb.count; ++e) {
+        const double a = -kTwoPi * double(tb.exponent(e) % tb.len) / double(tb.len);
+        w[e] = make_float2(float(std::cos(a)), float(std::sin(a)));
+    }
+    return w;
+}
+
 template <class F>
 int guarded(F&& body) {
     try {
@@ -372,8 +382,8 @@ void add_fused_series(brp_hip& h, const brp_params& p, float* x) {
     auto& a = s.tables;
     a.x = x;
     a.lut = h.lut;
-    a.tw_c = h.upload(twiddles(sh.c, sh.c));
-    a.tw_r = h.upload(twiddles(sh.r, sh.r));
+    a.tw_c = h.upload(stage_twiddles(brp_engine::fft_stage_table(sh.c, brp_engine::fused_pass1_elements(sh))));
+    a.tw_r = h.upload(stage_twiddles(brp_engine::fft_stage_table(sh.r, brp_engine::fused_pass2_elements(sh))));
     a.twm_lo = h.upload(twiddles(sh.m, 2048));
     a.twm_hi = h.upload(twiddles(sh.m, (sh.m + 2047) / 2048, 2048));
     a.tw_rq = h.upload(twiddles(16ull * sh.r, 16ull * sh.r));

@@ -94,6 +94,19 @@ constexpr bool plan_ok(int L, int e) {
     return pl.count > 0 && prod == L;
 }
 
+// Offset of stage ST's block in the per-stage twiddle table of an L-point transform at E
+// elements per thread (layout: brp_engine::fft_stage_table).
+template <int L, int E, int ST>
+constexpr int stage_table_offset() {
+    const Radices pl = plan_for(L, E);
+    int off = 0, ns = pl.r[0];
+    for (int i = 1; i < ST; ++i) {
+        off += (pl.r[i] - 1) * ns;
+        ns *= pl.r[i];
+    }
+    return off;
+}
+
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t grid) {
     if (grid % 8u) return bid;
     return (bid % 8u) * (grid / 8u) + bid / 8u;
@@ -111,7 +124,10 @@ constexpr int lds_size() { return (L + L / 16) * S; }
 // Stockham stage ST of an L-point FFT over S sequences held by T threads, E = L S / T
 // elements each.  On entry v[u R + q] holds input q of butterfly u = (s, j), b = t + T u,
 // s = b mod S, j = b / S, i.e. element j + q L/R.  On exit of the last stage v[u R + q]
-// holds output element j + q L/R of sequence s.
+// holds output element j + q L/R of sequence s.  tw is the transform's per-stage table:
+// input q of a butterfly with k = j mod NS takes W_L^{k q L / (NS R)} from entry
+// (q - 1) NS + k of the stage's block, so for each q a wave reads a run of consecutive
+// entries (one or two cache lines) rather than entries k q apart.
 // MF: radix-16 stages as split-FP16 MFMA GEMMs (the precision variant; wave-collective,
 // so every lane of the block runs the same stages).
 template <int L, int S, int T, int ST, bool MF = false>
@@ -129,7 +145,8 @@ __device__ __forceinline__ void fft_stages(cf* v, float2* lds, const float2* __r
         const int k = j % NS;
         if constexpr (NS > 1) {
 #pragma unroll
-            for (int q = 1; q < R; ++q) v[u * R + q] = mul(v[u * R + q], load(tw + k * q * (L / (NS * R))));
+            for (int q = 1; q < R; ++q)
+                v[u * R + q] = mul(v[u * R + q], load(tw + (stage_table_offset<L, E, ST>() + (q - 1) * NS + k)));
         }
         if constexpr (MF && R == 16) dft16_mfma(v + u * R, *mf);
         else dft<R>(v + u * R);
@@ -176,8 +193,7 @@ __device__ __forceinline__ uint32_t exchange_pos(uint32_t k2, uint32_t C) {
 // ------------------------------------------------------------------ pass 1 --
 // Resample + pack the E elements of thread t: element e (butterfly u = e / R0, input
 // q = e % R0) is the packed pair z[m] = (y[2m], y[2m+1]) with m = r0 + s + R c,
-// c = j + q C/R0.  In chunks of CH elements: every index of the chunk, then every load,
-// then the masking, so all of a thread's loads can be in flight together.  The two
+// c = j + q C/R0.  The two
 // samples of a pair come from source indices j0 <= j1 <= j0 + 2 (the orbit moves the
 // index by far less than one sample per sample), so one 12-byte load at
 // b = clamp(j0, 0, U - 1) serves both (the observation buffer holds zero guard samples
@@ -195,8 +211,9 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
     const uint32_t R = sh.r;
     const uint32_t steps = p.steps, U = p.unpadded;
     const Orbit32 o{p.tau, p.omega, p.psi, p.dt, p.step_inv, p.s0, p.flags};
+    int tm = t;  // the thread's index as element_m reads it
     auto element_m = [&](int e) -> uint32_t {
-        const int b = t + T * (e / R0);
+        const int b = tm + T * (e / R0);
         const int s = b % S, j = b / S;
         return r0 + s + __umul24(R, uint32_t(j + (e % R0) * (C / R0)));  // R, c < 2^24
     };
@@ -209,79 +226,157 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
             return source_index4<NN, FM>(i, o, lut4);
         }
     };
-    constexpr int CH = 2;  // elements per chunk: bounds the live registers of the gather
-    constexpr int K = E / CH;
-    static_assert(E % CH == 0, "chunking");
-    struct Trio { float v[3]; };
-    auto indices = [&](int k, int* src) {
+    // Validation runs: the pairs of elements [e0, e0 + count) and their source indices
+    // (computed again) for the samples below steps.
+    auto write_snapshot = [&](int e0, int count) {
+        if constexpr (SNAP) {
 #pragma unroll
-        for (int e = 0; e < CH; ++e) {
-            const uint32_t m = element_m(k * CH + e);
-            src[2 * e] = index_of(2 * m);
-            src[2 * e + 1] = index_of(2 * m + 1);
-        }
-    };
-    auto issue = [&](const int* src, Trio* w) {
-#pragma unroll
-        for (int e = 0; e < CH; ++e) {
-            const int b = min(max(src[2 * e], 0), int(U) - 1);
-            __builtin_memcpy(&w[e], a.x + b, sizeof(Trio));
-        }
-    };
-    auto consume = [&](int k, const int* src, const Trio* w) {
-        float y[2 * CH];
-        bool odd = false;
-#pragma unroll
-        for (int e = 0; e < CH; ++e) {
-            const int b = min(max(src[2 * e], 0), int(U) - 1);
-            const uint32_t d1 = uint32_t(src[2 * e + 1] - b);
-            y[2 * e] = w[e].v[0];
-            y[2 * e + 1] = d1 == 0u ? w[e].v[0] : (d1 == 1u ? w[e].v[1] : w[e].v[2]);
-            odd |= d1 > 2u;
-        }
-        if (odd) {  // a pair whose second index is not within two samples of the first
-#pragma unroll
-            for (int e = 0; e < CH; ++e) {
-                const int b = min(max(src[2 * e], 0), int(U) - 1);
-                if (uint32_t(src[2 * e + 1] - b) > 2u) y[2 * e + 1] = a.x[min(max(src[2 * e + 1], 0), int(U) - 1)];
+            for (int c = 0; c < count; ++c) {
+                const uint32_t i = 2 * element_m(e0 + c);
+                if (ALL || i < steps) {
+                    a.snap_resampled[size_t(slot) * a.snap_n + i] = v[e0 + c].x;
+                    a.snap_index[size_t(slot) * a.snap_n + i] = index_of(i);
+                }
+                if (ALL || i + 1 < steps) {
+                    a.snap_resampled[size_t(slot) * a.snap_n + i + 1] = v[e0 + c].y;
+                    a.snap_index[size_t(slot) * a.snap_n + i + 1] = index_of(i + 1);
+                }
             }
         }
+    };
+    // Up to 16 elements at once (all of them at 16 or fewer elements per thread; 24-element
+    // plans in two halves), so two selector bits per element fit one word.
+    constexpr int CH = E <= 16 ? E : E / 2;
+    static_assert(E % CH == 0 && CH <= 16, "gather chunks");
+    struct Trio { float v[3]; };
+    // Interior blocks (every sample below steps) first take the form without per-sample
+    // masks: per element only the selector bits, and per lane the smallest first and the
+    // largest second index.  When every lane of the wave stayed inside the observation, no
+    // sample needs masking; otherwise the wave starts again in the masked form below (rare:
+    // only orbits that reach past the observation's ends).  Both forms add the same values
+    // in the same order, so the sums are identical.
+    if constexpr (ALL && CH == E) {
+        Trio w[E];
+        uint32_t sel = 0;
+        int lo = 0x7fffffff, hi = -0x7fffffff;
 #pragma unroll
-        for (int e = 0; e < CH; ++e) {
-            const uint32_t m = element_m(k * CH + e);
-            float pair[2];
+        for (int e = 0; e < E; ++e) {
+            const uint32_t m = element_m(e);
+            const int j0 = index_of(2 * m), j1 = index_of(2 * m + 1);
+            const int b = min(max(j0, 0), int(U) - 1);
+            __builtin_memcpy(&w[e], reinterpret_cast<const char*>(a.x) + uint32_t(b) * 4u, sizeof(Trio));
+            sel |= min(uint32_t(j1 - b), 3u) << (2 * e);
+            lo = min(lo, j0);
+            hi = max(hi, j1);
+            asm volatile("" : "+v"(sel), "+v"(lo), "+v"(hi));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (__all(lo >= 0 && hi < int(U))) {
+            double sum = 0.0;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const uint32_t i = 2 * m + h;
-                const int jj = src[2 * e + h];
-                const bool in_series = ALL || i < steps;
-                const bool inside = uint32_t(jj) < U;
-                bad |= in_series && !inside;
-                pair[h] = (in_series && inside) ? y[2 * e + h] : 0.0f;
-                acc += double(pair[h]);
-                if constexpr (SNAP) {
-                    if (in_series) {
-                        a.snap_resampled[size_t(slot) * a.snap_n + i] = pair[h];
-                        a.snap_index[size_t(slot) * a.snap_n + i] = jj;
+            for (int e = 0; e < E; ++e) {
+                const uint32_t code = (sel >> (2 * e)) & 3u;
+                const uint32_t u0 = __builtin_bit_cast(uint32_t, w[e].v[0]);
+                const uint32_t u1 = __builtin_bit_cast(uint32_t, w[e].v[1]);
+                const uint32_t u2 = __builtin_bit_cast(uint32_t, w[e].v[2]);
+                const uint32_t b0 = 0u - (code & 1u), b1 = 0u - ((code >> 1) & 1u);
+                const uint32_t low = (u1 & b0) | (u0 & ~b0);
+                const uint32_t pick = ((u2 & b1) | (low & ~b1)) & (0u - uint32_t(code != 3u));
+                const float y0 = w[e].v[0], y1 = __builtin_bit_cast(float, pick);
+                sum += double(y0);
+                sum += double(y1);
+                v[e] = cf{y0, y1};
+            }
+            if (sel & (sel >> 1) & 0x55555555u) {  // selector 3: the second sample's own load
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if (((sel >> (2 * e)) & 3u) == 3u) {
+                        uint32_t mm = element_m(e);
+                        asm volatile("" : "+v"(mm));
+                        const int j1 = index_of(2 * mm + 1);
+                        const bool inside = uint32_t(j1) < U;
+                        bad |= !inside;
+                        const float y1 = inside ? a.x[j1] : 0.0f;
+                        sum += double(y1);
+                        v[e].y = y1;
                     }
                 }
             }
-            v[k * CH + e] = cf{pair[0], pair[1]};
+            acc += sum;
+            write_snapshot(0, E);
+            return;
         }
-    };
-    // Two chunks in flight: the indices and loads of chunk k + 1 are issued before the
-    // samples of chunk k are consumed.
-    int src[2][2 * CH];
-    Trio w[2][CH];
-    indices(0, src[0]);
-    issue(src[0], w[0]);
+        // The masked form from an opaque thread index: reusing the unmasked form's index
+        // arithmetic would keep all of it live through that form's gather.
+        asm volatile("" : "+v"(tm));
+    }
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (k + 1 < K) {
-            indices(k + 1, src[(k + 1) & 1]);
-            issue(src[(k + 1) & 1], w[(k + 1) & 1]);
+    for (int e0 = 0; e0 < E; e0 += CH) {
+        // Phase 1: per element, both source indices, then the 12-byte load at
+        // b = clamp(j0, 0, U - 1) at once, so every load of the chunk is in flight while the
+        // later indices are computed.  Kept per element besides the loaded trio: which word
+        // of it holds the second sample (0, 1, 2; 3 = neither, a second load below) and
+        // whether each sample lies in the series and inside the observation (else it is 0).
+        Trio w[CH];
+        uint32_t sel = 0, keep = 0, outside = 0;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const uint32_t m = element_m(e0 + c);
+            const int j0 = index_of(2 * m), j1 = index_of(2 * m + 1);
+            const int b = min(max(j0, 0), int(U) - 1);
+            // 32-bit byte offset from the observation's base (U < 2^30 samples)
+            __builtin_memcpy(&w[c], reinterpret_cast<const char*>(a.x) + uint32_t(b) * 4u, sizeof(Trio));
+            const uint32_t d1 = uint32_t(j1 - b);
+            sel |= min(d1, 3u) << (2 * c);
+            const bool s0 = ALL || 2 * m < steps, s1 = ALL || 2 * m + 1 < steps;
+            const bool in0 = uint32_t(j0) < U, in1 = uint32_t(j1) < U;
+            outside |= uint32_t((s0 && !in0) || (s1 && !in1));
+            keep |= (uint32_t(s0 && in0) | (uint32_t(s1 && in1) << 1)) << (2 * c);
+            // The element's bits are settled here (sinking their arithmetic into phase 2
+            // would keep both indices of every element live), and one element's index
+            // arithmetic runs at a time (interleaving them would keep every intermediate
+            // live beside the loads in flight).
+            asm volatile("" : "+v"(sel), "+v"(keep), "+v"(outside));
+            __builtin_amdgcn_sched_barrier(0);
         }
-        consume(k, src[k & 1], w[k & 1]);
+        bad |= outside != 0u;
+        // Phase 2: the pairs from the trios, masked, and the FP64 sum of the series samples
+        // in element order.
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            // Bitwise selects (v_bfi / v_and): a select between the trio's words by the
+            // selector would become an indexed array in scratch memory.
+            const uint32_t code = (sel >> (2 * c)) & 3u;
+            const uint32_t k = keep >> (2 * c);
+            const uint32_t u0 = __builtin_bit_cast(uint32_t, w[c].v[0]);
+            const uint32_t u1 = __builtin_bit_cast(uint32_t, w[c].v[1]);
+            const uint32_t u2 = __builtin_bit_cast(uint32_t, w[c].v[2]);
+            const uint32_t b0 = 0u - (code & 1u), b1 = 0u - ((code >> 1) & 1u);
+            const uint32_t low = (u1 & b0) | (u0 & ~b0);  // selector 0 or 1
+            const uint32_t pick = (u2 & b1) | (low & ~b1);
+            const uint32_t keep0 = 0u - (k & 1u), keep1 = 0u - (((k >> 1) & 1u) & uint32_t(code != 3u));
+            const float y0 = __builtin_bit_cast(float, u0 & keep0);
+            const float y1 = __builtin_bit_cast(float, pick & keep1);
+            acc += double(y0);
+            acc += double(y1);
+            v[e0 + c] = cf{y0, y1};
+        }
+        // Rare: a second index not within two samples of the first (selector 3) takes its
+        // own load.  The index is computed again from an opaque m: reusing the first
+        // computation would keep every element's second index live through the gather.
+        if (sel & (sel >> 1) & 0x55555555u) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                if (((sel >> (2 * c)) & 3u) == 3u && ((keep >> (2 * c)) & 2u)) {
+                    uint32_t mm = element_m(e0 + c);
+                    asm volatile("" : "+v"(mm));
+                    const float y1 = a.x[min(max(index_of(2 * mm + 1), 0), int(U) - 1)];
+                    acc += double(y1);
+                    v[e0 + c].y = y1;
+                }
+            }
+        }
+        write_snapshot(e0, CH);
     }
 }
 
@@ -321,15 +416,21 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 
         resample_pairs<C, S, T, SNAP, F64, NN, FM, true>(v, acc, bad, a, sh, p, lut4, r0, slot, t);
     else
         resample_pairs<C, S, T, SNAP, F64, NN, FM, false>(v, acc, bad, a, sh, p, lut4, r0, slot, t);
+    // Keep the FFT's table reads out of the gather: hoisted there, their registers would
+    // sit beside the gather's loads in flight.
+    __builtin_amdgcn_sched_barrier(0);
     fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
 
     // Y[r][k2] = v * W_M^{r k2} for k2 = j + q C/RL: W_M^{r j} times
     // W_M^{r q C/RL} = W_{R RL}^{r q} (r q < R RL, read from the W_{16 R} table), all of a
-    // thread's table reads issued together.  Column k2 is stored at exchange_pos(k2, C):
-    // in the mirror layout 2 k2 below C/2 (from lo) and 2 (C - k2) + 1 above (from hi),
-    // C/2 at 1.
-    float2* Y = a.Y + size_t(g) * M;
+    // thread's table reads issued together.  Column k2 goes to position exchange_pos(k2, C)
+    // of its row (in the mirror layout 2 k2 below C/2, 2 (C - k2) + 1 above, C/2 at 1).
+    // The block's S rows are assembled in LDS first and then written as 16-byte pieces of
+    // consecutive positions, so every store instruction of a wave fills whole lines (the
+    // mirror positions taken straight from registers would half-fill twice as many).
     const bool mirror = sh.mirror != 0;
+    float2* stage = lds;  // S rows of C positions: S C <= lds_size<C, S>()
+    __syncthreads();      // the last FFT stage's LDS reads are done
 #pragma unroll
     for (int u = 0; u < E / RL; ++u) {
         const int b = t + T * u;
@@ -339,24 +440,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 
 #pragma unroll
         for (int q = 1; q < RL; ++q) pw[q] = load(a.tw_rq + __umul24(r, uint32_t(q * (16 / RL))));
         const cf base = twiddle2(a.twm_lo, a.twm_hi, __umul24(r, uint32_t(j)));
-        float2* row = Y + size_t(r) * C;
-        if (mirror) {
-            float2* lo = row + 2 * j;
-            float2* hi = row + (2 * C + 1 - 2 * j);
+        float2* row = stage + s * C;
 #pragma unroll
-            for (int q = 0; q < RL; ++q) {
-                const cf w = q == 0 ? base : mul(base, pw[q]);
-                float2* dst = q < RL / 2 ? lo + 2 * q * (C / RL)
-                                         : ((q == RL / 2 && j == 0) ? row + 1 : hi - 2 * q * (C / RL));
-                store(dst, mul(v[u * RL + q], w));
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < RL; ++q) {
-                const cf w = q == 0 ? base : mul(base, pw[q]);
-                store(row + j + q * (C / RL), mul(v[u * RL + q], w));
-            }
+        for (int q = 0; q < RL; ++q) {
+            const cf w = q == 0 ? base : mul(base, pw[q]);
+            const int k2 = j + q * (C / RL);
+            const int pos = !mirror ? k2
+                            : (q < RL / 2 ? 2 * k2 : ((q == RL / 2 && j == 0) ? 1 : 2 * (C - k2) + 1));
+            store(row + pos, mul(v[u * RL + q], w));
         }
+    }
+    __syncthreads();
+    {
+        static_assert((S * C) % (2 * T) == 0, "16-byte row pieces per thread");
+        float4* Y4 = reinterpret_cast<float4*>(a.Y + size_t(g) * M + size_t(r0) * C);  // rows r0 .. r0 + S - 1
+        const float4* st4 = reinterpret_cast<const float4*>(stage);
+#pragma unroll
+        for (int k = 0; k < S * C / (2 * T); ++k) Y4[t + T * k] = st4[t + T * k];
     }
 
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
@@ -923,7 +1023,7 @@ bool with_pass1(uint32_t c, uint32_t s1, uint32_t t1, F&& f) {
     switch (inst(c, s1, t1)) {
 #define BRP_P1(C, S, T) case inst(C, S, T): f.template operator()<C, S, T>(); return true;
         BRP_P1(64, 64, 256) BRP_P1(128, 32, 256) BRP_P1(256, 16, 256) BRP_P1(512, 16, 512)
-        BRP_P1(1024, 16, 1024) BRP_P1(2048, 8, 1024) BRP_P1(2048, 4, 512) BRP_P1(2048, 2, 256) BRP_P1(2048, 2, 512)
+        BRP_P1(1024, 16, 1024) BRP_P1(1024, 8, 512) BRP_P1(2048, 8, 1024) BRP_P1(2048, 4, 512) BRP_P1(2048, 2, 256) BRP_P1(2048, 2, 512)
         BRP_P1(4096, 4, 1024) BRP_P1(4096, 2, 512) BRP_P1(4096, 1, 256)
         BRP_P1(4096, 2, 1024) BRP_P1(4096, 1, 512)
         BRP_P1(384, 32, 512) BRP_P1(768, 16, 512) BRP_P1(1536, 8, 512) BRP_P1(3072, 2, 256)
@@ -937,7 +1037,7 @@ bool with_pass2(uint32_t r, uint32_t s2, uint32_t t2, F&& f) {
 #define BRP_P2(R, S, T) case inst(R, S, T): f.template operator()<R, S, T>(); return true;
         BRP_P2(64, 32, 256) BRP_P2(128, 16, 256) BRP_P2(256, 16, 512) BRP_P2(256, 8, 256)
         BRP_P2(512, 16, 1024) BRP_P2(512, 8, 512) BRP_P2(512, 4, 256) BRP_P2(512, 4, 512)
-        BRP_P2(1024, 8, 1024) BRP_P2(1024, 4, 512) BRP_P2(1024, 2, 256) BRP_P2(2048, 4, 1024)
+        BRP_P2(1024, 8, 1024) BRP_P2(1024, 4, 512) BRP_P2(1024, 2, 256) BRP_P2(2048, 4, 1024) BRP_P2(2048, 2, 512)
         BRP_P2(192, 16, 256) BRP_P2(384, 8, 256) BRP_P2(768, 4, 256) BRP_P2(1536, 2, 256)
         BRP_P2(384, 16, 512) BRP_P2(768, 8, 512)
 #undef BRP_P2
@@ -994,6 +1094,8 @@ constexpr ShapeRow kShapes[] = {
     {1u << 21, 4096, 512, 2, 4, 1024, 256},    {1u << 21, 4096, 512, 1, 4, 512, 256},
     {1u << 21, 4096, 512, 2, 4, 512, 512},     {1u << 21, 4096, 512, 1, 4, 512, 512},
     {1u << 21, 2048, 1024, 2, 4, 512, 512},
+    // eight 1024-point rows per pass-1 block (64-byte pieces of each observation line)
+    {1u << 21, 1024, 2048, 8, 2, 512, 512},
 };
 
 }  // namespace
@@ -1001,6 +1103,25 @@ constexpr ShapeRow kShapes[] = {
 namespace brp_engine {
 
 bool fused_mfma_capable(const FusedShape& s) { return plan_for(int(s.r), int(2 * s.r * s.s2 / s.t2)).r[0] == 16; }
+
+StageTable fft_stage_table(uint32_t len, uint32_t elements) {
+    const Radices pl = plan_for(int(len), int(elements));
+    StageTable tb;
+    uint32_t ns = uint32_t(pl.r[0]);
+    for (int st = 1; st < pl.count; ++st) {
+        tb.offset[tb.stages] = tb.count;
+        tb.ns[tb.stages] = ns;
+        tb.radix[tb.stages] = uint32_t(pl.r[st]);
+        tb.count += (uint32_t(pl.r[st]) - 1) * ns;
+        ns *= uint32_t(pl.r[st]);
+        ++tb.stages;
+    }
+    tb.len = len;
+    return tb;
+}
+
+uint32_t fused_pass1_elements(const FusedShape& s) { return s.c * s.s1 / s.t1; }
+uint32_t fused_pass2_elements(const FusedShape& s) { return 2 * s.r * s.s2 / s.t2; }
 
 namespace {
 // The shape of a table row, if both of its kernels are compiled.

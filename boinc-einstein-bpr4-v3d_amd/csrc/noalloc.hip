@@ -77,11 +77,13 @@ __global__ void k_setup(SetupArgs s) {
     if (t < 65) s.lut_out[t] = s.lut[t];
 }
 
-// W_len^t for t = (i stride) mod len, i < count, of each table: double cos / sin of
+// W_len^t for t = (i stride) mod len, i < count, of each table (or, for the FFT passes'
+// per-stage tables, t = the stage table's exponent of entry i): double cos / sin of
 // -2 pi t / len rounded once to float.
 struct Table {
     float2* out;
     uint64_t len, count, stride;
+    brp_engine::StageTable stage;  // stage.count > 0: a per-stage table
 };
 constexpr int kTables = 7;
 struct TableArgs {
@@ -96,7 +98,7 @@ __global__ void k_twiddles(TableArgs a) {
     while (i >= a.start[k + 1]) ++k;
     const Table& tb = a.table[k];
     const uint64_t e = i - a.start[k];
-    const uint64_t t = (e * tb.stride) % tb.len;
+    const uint64_t t = (tb.stage.count ? tb.stage.exponent(uint32_t(e)) : e * tb.stride) % tb.len;
     const double ang = -6.283185307179586476925286766559 * double(t) / double(tb.len);
     tb.out[e] = make_float2(float(cos(ang)), float(sin(ang)));
 }
@@ -150,14 +152,18 @@ extern "C" int brp_hip_search_noalloc(const brp_params* templates, size_t count,
         brp_params* params = cv.take<brp_params>(count);
         float2* lut = cv.take<float2>(65);
         TableArgs ta{};
+        const brp_engine::StageTable st1 = brp_engine::fft_stage_table(sh.c, brp_engine::fused_pass1_elements(sh));
+        const brp_engine::StageTable st2 = brp_engine::fft_stage_table(sh.r, brp_engine::fused_pass2_elements(sh));
         const uint64_t lens[kTables] = {sh.c, sh.r, sh.m, sh.m, 16ull * sh.r, sh.n, sh.n};
-        const uint64_t counts[kTables] = {sh.c, sh.r, 2048, (sh.m + 2047) / 2048, 16ull * sh.r, 2048,
+        const uint64_t counts[kTables] = {st1.count, st2.count, 2048, (sh.m + 2047) / 2048, 16ull * sh.r, 2048,
                                           (sh.n + 2047) / 2048};
         const uint64_t strides[kTables] = {1, 1, 1, 2048, 1, 1, 2048};
         for (int k = 0; k < kTables; ++k) {
-            ta.table[k] = Table{cv.take<float2>(counts[k]), lens[k], counts[k], strides[k]};
+            ta.table[k] = Table{cv.take<float2>(counts[k]), lens[k], counts[k], strides[k], {}};
             ta.start[k + 1] = ta.start[k] + counts[k];
         }
+        ta.table[0].stage = st1;
+        ta.table[1].stage = st2;
         double* partial = cv.take<double>(count * sh.blocks1);
         float* mean = cv.take<float>(count);
         float2* Y = cv.take<float2>(count * size_t(sh.m));

@@ -723,6 +723,41 @@ def test_out_of_range_index_flagged(brp, wl):
                 g.candidates(0)
 
 
+@pytest.mark.parametrize("psi", [2.5, 0.7])
+def test_out_of_range_indices_in_snapshot(brp, oracle, wl, psi):
+    """Source indices past the observation inside blocks whose samples all lie below steps:
+    the fused engine's unmasked gather gives way to the masked one for the waves that leave
+    the observation, which writes the reference's index rule for every sample
+    (src/science.cpp:80) and zero for the samples outside.  psi = 2.5 runs past the end,
+    psi = 0.7 stays inside (the unmasked form throughout)."""
+    w = wl.plain_small()
+    p = brp.parameters(w.n, w.n, 0.05, 3600.0, psi, wl.DT)
+    p.steps = p.unpadded
+    lib, op = oracle.load(), oracle.as_oparams(p)
+    off = np.array([lib.oracle_sample_offset(i, op) for i in range(p.steps)], dtype=np.float32)
+    want_idx = np.trunc((np.arange(p.steps, dtype=np.float32) - off).astype(np.float64) + 0.5).astype(np.int32)
+    inside = (want_idx >= 0) & (want_idx < p.unpadded)
+    assert (not inside.all()) == (psi == 2.5)
+    want = np.where(inside, w.x[np.clip(want_idx, 0, p.unpadded - 1)], 0.0).astype(np.float32)
+    say(f"[gpu] fused snapshot indices with psi={psi} ({int((~inside).sum())} outside)")
+    with brp.GPU(p, w.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True) as g:
+        g.set_template(p)
+        g.run()
+        res = np.empty(p.n, np.float32)
+        idx = np.empty(p.steps, np.int32)
+        import ctypes
+        rc = brp.load().brp_hip_snapshot(g._h, 0, res.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), None, None, None)
+        assert rc == 0
+        if psi == 2.5:
+            with pytest.raises(RuntimeError, match="invalid resample index"):
+                g.candidates(0)
+        else:
+            g.candidates(0)
+    assert np.array_equal(idx, want_idx)
+    assert np.array_equal(res[:p.steps], want)
+
+
 def test_template_dimension_change_rejected(brp, wl):
     """The plain engine keeps the reference's fixed-dimension rule; the fused engine lifts
     it for F and H and for registered series, and still rejects an unregistered length,
