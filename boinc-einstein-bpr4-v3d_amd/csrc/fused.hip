@@ -582,19 +582,23 @@ k_pass2(FusedArgs a, FusedShape sh) {
     const uint32_t k_first = uint32_t(t / S) * C + base + uint32_t(t % S);
     cf we = twiddle2(a.twn_lo, a.twn_hi, mulmod_n(k_first, steps_n, sh));
     const cf wstep = twiddle2(a.twn_lo, a.twn_hi, mulmod_n(uint32_t(T / S) * C, steps_n, sh));
-    // Every W_N^k of the thread's pairs first, so their table reads are in flight together.
+    // W_N^k of the thread's pairs: pair u lies (T/S) u rows after pair 0, so
+    // W_N^{k_u} = W_N^{k_0} W_N^{(T/S) C u}, whose second factor is the same for every thread
+    // (scalar table reads).
     constexpr int PAIRS = R * S / T;
     cf Wk[PAIRS];
+    Wk[0] = twiddle2(a.twn_lo, a.twn_hi, k_first);
 #pragma unroll
-    for (int u = 0; u < PAIRS; ++u) {
-        const int pi = t + T * u;
-        Wk[u] = twiddle2(a.twn_lo, a.twn_hi, uint32_t(pi / S) * C + base + uint32_t(pi % S));
-    }
-    // X[k] and X[M - k] from Z[k] and Z[M - k] (zk, zm), W = W_N^k, we = W_N^{k steps}.
+    for (int u = 1; u < PAIRS; ++u)
+        Wk[u] = mul(Wk[0], twiddle2(a.twn_lo, a.twn_hi, uint32_t(u) * uint32_t(T / S) * C));
+    // X[k] and X[M - k] from Z[k] and Z[M - k] (zk, zm), W = W_N^k, we = W_N^{k steps}:
+    // X[k] = (A - i W B) / 2 and X[M - k] = conj(A + i W B) / 2 with A = Z_k + conj Z_{M-k} and
+    // B = Z_k - conj Z_{M-k} (as W_N^{M-k} = -conj W_N^k), so one product W B serves both.
     auto real_split = [&](const cf zk, const cf zm, const cf W, const cf we_k, uint32_t k, cf& xk, cf& xm) {
-        const cf Wm = cf{-W.x, W.y};  // W_N^{M-k} = -conj(W_N^k)
-        xk = scale(add(add(zk, conj(zm)), mul_negi(mul(sub(zk, conj(zm)), W))), 0.5f);
-        xm = scale(add(add(zm, conj(zk)), mul_negi(mul(sub(zm, conj(zk)), Wm))), 0.5f);
+        const cf A = add(zk, conj(zm)), B = sub(zk, conj(zm));
+        const cf WB = mul(B, W);
+        xk = cf{(A.x + WB.y) * 0.5f, (A.y - WB.x) * 0.5f};
+        xm = cf{(A.x - WB.y) * 0.5f, (-A.y - WB.x) * 0.5f};
         if (k == 0) {
             xk.x += mu * pad_len;
             xm.x += mu * (parity - 1.0f) * 0.5f;  // U[M] = ((-1)^steps - 1) / 2
