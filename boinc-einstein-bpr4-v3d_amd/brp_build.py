@@ -31,7 +31,10 @@ CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextr
 HIP_SOURCES = ["plain.hip", "fused.hip", "preprocess.hip", "toplist.hip", "engine.hip", "noalloc.hip"]
 CLI_SOURCES = ["brp_bench.cpp", "brp_workunit.cpp"]
 BIN = PKG / "bin"
-CXX_SOURCES = ["science.cpp"]
+CXX_SOURCES = ["science.cpp", "counters.cpp"]
+# counters.cpp uses the rocprofiler-sdk headers (which include the HIP runtime headers) for its
+# types only: the library loads rocprofiler-sdk itself at run time, when asked to.
+CXX_EXTRA = {"counters.cpp": ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]}
 
 
 def _run(cmd: list[str]) -> None:
@@ -46,7 +49,7 @@ def build(verbose: bool = False) -> pathlib.Path:
     for src in HIP_SOURCES:
         jobs.append([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(BUILD / (src + ".o"))])
     for src in CXX_SOURCES:
-        jobs.append(["g++", *CXX_FLAGS, "-c", str(CSRC / src), "-o", str(BUILD / (src + ".o"))])
+        jobs.append(["g++", *CXX_FLAGS, *CXX_EXTRA.get(src, []), "-c", str(CSRC / src), "-o", str(BUILD / (src + ".o"))])
     workers = min(len(jobs), max(1, (os.cpu_count() or 4)))
     with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
         for cmd, fut in [(c, pool.submit(_run, c)) for c in jobs]:
@@ -55,7 +58,7 @@ def build(verbose: bool = False) -> pathlib.Path:
                 print(" ".join(cmd[:1] + cmd[-3:]))
     objects = [str(BUILD / (s + ".o")) for s in HIP_SOURCES + CXX_SOURCES]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objects, "-lpthread"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objects, "-lpthread", "-ldl"])
     os.replace(tmp, LIB)
     build_cli()
     return LIB

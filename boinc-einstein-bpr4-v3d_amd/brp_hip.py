@@ -447,8 +447,13 @@ class GPU:
         return int(load().brp_hip_allocated_bytes(self._h))
 
     def profile_counters(self) -> str:
-        buf = ctypes.create_string_buffer(1024)
-        load().brp_hip_profile_counters(self._h, buf, 1024)
+        """Per-kernel hardware counters of the last batch as JSON (collected on each call; in
+        process only when BRP_HIP_COUNTERS=1 was set before the library loaded)."""
+        cap = 1 << 20
+        buf = ctypes.create_string_buffer(cap)
+        n = load().brp_hip_profile_counters(self._h, buf, cap)
+        if n >= cap:
+            raise RuntimeError(f"profile_counters: {n} bytes of JSON exceed the {cap}-byte buffer")
         return buf.value.decode()
 
     def kernel_ms(self) -> tuple[list[float], int, int]:

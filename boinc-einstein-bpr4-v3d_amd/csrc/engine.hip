@@ -16,7 +16,12 @@
 #include <string>
 #include <vector>
 
+#include <cxxabi.h>
+
+#include <map>
+
 #include "brp_hip.h"
+#include "counters.h"
 #include "engine.h"
 #include "errors.h"
 #include "lut.h"
@@ -901,10 +906,162 @@ size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap) {
 
 uint64_t brp_hip_allocated_bytes(const brp_hip* h) { return h ? h->allocated : 0; }
 
+}  // extern "C"
+
+namespace {
+// ------------------------------------------------------------- counters --
+std::string json_quote(const std::string& text) {
+    std::string out = "\"";
+    for (char c : text) {
+        if (c == '"' || c == '\\') out += '\\';
+        if (c == '\n') out += "\\n";
+        else if (uint8_t(c) >= 0x20) out += c;
+    }
+    return out + "\"";
+}
+
+// The kernel's demangled name without its namespace and parameter list.
+std::string kernel_label(const std::string& symbol) {
+    int status = 0;
+    char* d = abi::__cxa_demangle(symbol.c_str(), nullptr, nullptr, &status);
+    std::string name = (status == 0 && d) ? d : symbol;
+    std::free(d);
+    // Function templates demangle with their return type first ("void (anonymous
+    // namespace)::k_pass1<...>(...)").
+    if (name.compare(0, 5, "void ") == 0) name = name.substr(5);
+    for (const char* ns : {"(anonymous namespace)::", "brp_engine::"})
+        for (size_t at; (at = name.find(ns)) != std::string::npos;) name.erase(at, std::strlen(ns));
+    int depth = 0;
+    for (size_t i = 0; i < name.size(); ++i) {  // cut the parameter list at depth 0
+        if (name[i] == '<') ++depth;
+        else if (name[i] == '>') --depth;
+        else if (name[i] == '(' && depth == 0) return name.substr(0, i);
+    }
+    return name;
+}
+
+// The reference's stage of a kernel (src/vulkan.cpp:221-266; the fused kernels span stages).
+std::string kernel_stage(const std::string& label) {
+    auto starts = [&](const char* p) { return label.compare(0, std::strlen(p), p) == 0; };
+    if (starts("k_pass1")) return "pass1_resample_prepare_fft";
+    if (starts("k_pass2")) return "pass2_fft_power";
+    if (starts("k_harmonic") || starts("k_hs_")) return "harmonic_candidates";
+    if (starts("k_resample") || starts("k_mean") || starts("k_pack<") || label == "k_pack") return "resample_prepare";
+    if (starts("k_stockham")) return "fft";
+    if (starts("k_real") || starts("k_power")) return "power";
+    if (starts("brp_copy")) return "calibration_copy";
+    return "other";
+}
+
+// profile_counters (src/vulkan.cpp:346-406) in process: the handle's last batch replayed once
+// per counter set under rocprofiler-sdk's dispatch counting service, with a plain copy of known
+// size in the same replay that calibrates FETCH_SIZE / WRITE_SIZE to bytes.  Each set fits one
+// hardware pass on gfx950 (at most 8 SQ, 4 TCC, 4 TCP, 2 TA, 2 TD, 2 GRBM counters).
+std::string profile_counters_json(brp_hip* h) {
+    const std::string why = brp_counters::unavailable_reason();
+    if (!why.empty()) return "{\"available\":false,\"reason\":" + json_quote(why) + "}";
+    require(h, "null handle");
+    require(h->ran[0], "run() has not been called");
+    const std::vector<std::vector<std::string>> sets = {
+        {"SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR", "SQ_INSTS_LDS", "SQ_WAVE_CYCLES",
+         "SQ_WAIT_INST_ANY", "GRBM_GUI_ACTIVE", "FETCH_SIZE"},
+        {"WRITE_SIZE", "TA_TA_BUSY_sum", "TD_TD_BUSY_sum", "TCP_TOTAL_CACHE_ACCESSES_sum", "TCC_HIT_sum", "TCC_MISS_sum"}};
+    const size_t copy_bytes = size_t(64) << 20;
+    void* src = nullptr;
+    void* dst = nullptr;
+    HIPC(hipMalloc(&src, copy_bytes));
+    HIPC(hipMalloc(&dst, copy_bytes));
+    struct Agg {
+        std::string stage;
+        uint64_t dispatches = 0;
+        double us = 0;
+        std::map<std::string, double> sum;
+    };
+    std::map<std::string, Agg> kernels;
+    std::vector<std::string> missing, gone;
+    try {
+        for (const auto& set : sets) {
+            const auto ds = brp_counters::collect(
+                set,
+                [&] {
+                    launch(*h, 0);
+                    finish(*h, 0);
+                    brp_engine::copy16(h->stream, src, dst, copy_bytes);
+                    HIPC(hipStreamSynchronize(h->stream));
+                },
+                &gone);
+            missing.insert(missing.end(), gone.begin(), gone.end());
+            std::map<std::string, uint64_t> seen;
+            for (const auto& d : ds) {
+                const std::string label = kernel_label(d.kernel);
+                Agg& a = kernels[label];
+                a.stage = kernel_stage(label);
+                if (&set == &sets[0]) {
+                    ++a.dispatches;
+                    a.us += double(d.end_ns - d.start_ns) * 1e-3;
+                }
+                for (const auto& kv : d.values) a.sum[kv.first] += kv.second;
+            }
+        }
+    } catch (...) {
+        hipFree(src);
+        hipFree(dst);
+        throw;
+    }
+    HIPC(hipFree(src));
+    HIPC(hipFree(dst));
+    // Counter KiB -> bytes, calibrated on the copy (which reads and writes copy_bytes).
+    double fetch_factor = 0, write_factor = 0;
+    for (const auto& kv : kernels) {
+        if (kv.second.stage != "calibration_copy" || !kv.second.dispatches) continue;
+        const auto& m = kv.second.sum;
+        const double n = double(kv.second.dispatches);
+        if (m.count("FETCH_SIZE") && m.at("FETCH_SIZE") > 0) fetch_factor = double(copy_bytes) / (m.at("FETCH_SIZE") / n);
+        if (m.count("WRITE_SIZE") && m.at("WRITE_SIZE") > 0) write_factor = double(copy_bytes) / (m.at("WRITE_SIZE") / n);
+    }
+    std::ostringstream out;
+    out.precision(10);
+    out << "{\"available\":true,\"source\":\"rocprofiler-sdk dispatch counting service\",\"passes\":" << sets.size()
+        << ",\"templates\":" << h->templates[0].size() << ",\"calibration\":{\"copy_bytes\":" << copy_bytes
+        << ",\"fetch_bytes_per_unit\":" << fetch_factor << ",\"write_bytes_per_unit\":" << write_factor << "},\"missing\":[";
+    for (size_t i = 0; i < missing.size(); ++i) out << (i ? "," : "") << json_quote(missing[i]);
+    out << "],\"stages\":[";
+    bool first = true;
+    for (const auto& kv : kernels) {
+        const Agg& a = kv.second;
+        if (!a.dispatches) continue;
+        const double n = double(a.dispatches);
+        out << (first ? "" : ",") << "{\"stage\":" << json_quote(a.stage) << ",\"kernel\":" << json_quote(kv.first)
+            << ",\"dispatches\":" << a.dispatches << ",\"counters\":{";
+        first = false;
+        bool f2 = true;
+        for (const auto& c : a.sum) {
+            out << (f2 ? "" : ",") << json_quote(c.first) << ":" << c.second / n;
+            f2 = false;
+        }
+        out << "},\"derived\":{\"duration_us\":" << a.us / n;
+        auto get = [&](const char* name) { return a.sum.count(name) ? a.sum.at(name) / n : -1.0; };
+        if (get("FETCH_SIZE") >= 0 && fetch_factor > 0) out << ",\"read_bytes\":" << get("FETCH_SIZE") * fetch_factor;
+        if (get("WRITE_SIZE") >= 0 && write_factor > 0) out << ",\"write_bytes\":" << get("WRITE_SIZE") * write_factor;
+        if (get("SQ_WAVES") > 0 && get("SQ_INSTS_VALU") >= 0)
+            out << ",\"valu_per_wave\":" << get("SQ_INSTS_VALU") / get("SQ_WAVES");
+        out << "}}";
+    }
+    out << "]}";
+    return out.str();
+}
+
+}  // namespace
+
+extern "C" {
+
 size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap) {
-    (void)h;
-    const std::string text =
-        "{\"available\":false,\"reason\":\"hardware counters are collected out of process with rocprofv3 --pmc\"}";
+    std::string text;
+    try {
+        text = profile_counters_json(h);
+    } catch (const std::exception& e) {
+        text = "{\"available\":false,\"reason\":" + json_quote(e.what()) + "}";
+    }
     if (buf && cap) {
         const size_t n = std::min(cap - 1, text.size());
         std::memcpy(buf, text.data(), n);

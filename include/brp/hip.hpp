@@ -17,6 +17,7 @@
 #include "brp/hip_science.hpp"
 #endif
 
+#include <algorithm>
 #include <array>
 #include <complex>
 #include <cstddef>
@@ -139,7 +140,18 @@ public:
     }
     std::string device_info() const { return text(brp_hip_device_info); }
     uint64_t allocated_bytes() const { return brp_hip_allocated_bytes(handle_.get()); }
-    std::string profile_counters() { return text(brp_hip_profile_counters); }
+    // Collected on every call (one replay of the batch per counter set), so the text is read in
+    // one call into a buffer that is grown and read again only when it was too small.
+    std::string profile_counters() {
+        std::string s(size_t(1) << 20, '\0');
+        size_t n = brp_hip_profile_counters(handle_.get(), s.data(), s.size());
+        if (n >= s.size()) {
+            s.assign(n + 1, '\0');
+            n = brp_hip_profile_counters(handle_.get(), s.data(), s.size());
+        }
+        s.resize(std::min(n, s.size() - 1));
+        return s;
+    }
 
     // Extensions (fused engine): the C handle, another observation on it (templates name
     // their series by (n, unpadded)), and the bank loop over prepared templates of any

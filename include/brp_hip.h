@@ -211,7 +211,14 @@ int brp_hip_candidates(brp_hip* h, uint32_t slot, brp_candidate* out, size_t cap
  * 5*fundamental floats. */
 int brp_hip_snapshot(brp_hip* h, uint32_t slot, float* resampled, int32_t* index, float* fft,
                      float* power, float* summed);
-/* device_info / allocated_bytes / profile_counters -- src/vulkan.cpp:338-406. */
+/* device_info / allocated_bytes / profile_counters -- src/vulkan.cpp:338-406.  Each writes at
+ * most cap - 1 characters and a terminating NUL, and returns the full length.
+ * profile_counters: JSON of per-kernel hardware counters, collected on every call by replaying
+ * the handle's last batch once per counter set under rocprofiler-sdk's dispatch counting service
+ * (with a plain copy of known size that calibrates FETCH_SIZE / WRITE_SIZE to bytes).  Needs
+ * BRP_HIP_COUNTERS=1 in the environment when the library loads (before the ROCm runtime starts
+ * in the process); otherwise, or when another tool such as rocprofv3 holds the profiler, the
+ * JSON is {"available": false, "reason": ...}.  h may be NULL for that availability answer. */
 size_t brp_hip_device_info(brp_hip* h, char* buf, size_t cap);
 uint64_t brp_hip_allocated_bytes(const brp_hip* h);
 size_t brp_hip_profile_counters(brp_hip* h, char* buf, size_t cap);
