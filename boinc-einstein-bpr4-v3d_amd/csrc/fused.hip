@@ -226,6 +226,12 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
             return source_index4<NN, FM>(i, o, lut4);
         }
     };
+    // The unmasked form's index: equal to index_of whenever it lies in [0, U) (U < 2^24), the
+    // only case in which that form keeps its result.
+    auto index_fast = [&](uint32_t i) -> int {
+        if constexpr (F64) return index_of(i);
+        else return source_index4<NN, FM, true>(i, o, lut4);
+    };
     // Validation runs: the pairs of elements [e0, e0 + count) and their source indices
     // (computed again) for the samples below steps.
     auto write_snapshot = [&](int e0, int count) {
@@ -262,10 +268,12 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint32_t m = element_m(e);
-            const int j0 = index_of(2 * m), j1 = index_of(2 * m + 1);
-            const int b = min(max(j0, 0), int(U) - 1);
-            __builtin_memcpy(&w[e], reinterpret_cast<const char*>(a.x) + uint32_t(b) * 4u, sizeof(Trio));
-            sel |= min(uint32_t(j1 - b), 3u) << (2 * e);
+            const int j0 = index_fast(2 * m), j1 = index_fast(2 * m + 1);
+            // An address inside the observation for any j0 (a negative j0 reads at U - 1 and
+            // sends the wave to the masked form anyway).
+            const uint32_t b = min(uint32_t(j0), U - 1);
+            __builtin_memcpy(&w[e], reinterpret_cast<const char*>(a.x) + b * 4u, sizeof(Trio));
+            sel |= min(uint32_t(j1) - b, 3u) << (2 * e);
             lo = min(lo, j0);
             hi = max(hi, j1);
             asm volatile("" : "+v"(sel), "+v"(lo), "+v"(hi));
