@@ -780,3 +780,18 @@ def test_template_dimension_change_rejected(brp, wl):
                 g.set_templates([p, bad])
         with pytest.raises(RuntimeError, match="batch exceeds"):
             g.set_templates([p, p, p])
+
+
+def test_rpi_rounding_matches_index_rule():
+    """The unmasked gather rounds with v_cvt_rpi_i32_f32; it must equal the index rule's
+    int(double(x) + 0.5) (src/science.cpp:80) for every float in [-0.5, 2^24]: an exhaustive
+    device check of 2.3e9 values (tests/probes/rpi_rounding.hip, built by build())."""
+    import pathlib
+    import subprocess
+    probe = pathlib.Path(__file__).resolve().parent / "probes" / "rpi_rounding"
+    if not probe.exists():
+        pytest.skip("probe not built (__graft_entry__.build() compiles it)")
+    say("[gpu] exhaustive v_cvt_rpi_i32_f32 check")
+    proc = subprocess.run([str(probe)], capture_output=True, text=True, timeout=120)
+    say(proc.stdout.strip())
+    assert proc.returncode == 0 and "0 mismatches" in proc.stdout.splitlines()[-1], proc.stdout + proc.stderr
