@@ -274,15 +274,17 @@ def main() -> int:
 
     # ---------------------------------------------------------- precision leg --
     # 3 * 2^20 (radix-3 plans), bench semantics, one batch each over the same templates:
-    # the split-FP16 MFMA variant (its shape needs a radix-16 pass-2 plan), the FP32 engine
-    # on that same shape, and the FP32 engine on its default shape.
+    # the split-FP16 MFMA variant on pass 2's radix-16 stages (its shape needs a radix-16
+    # pass-2 plan) and the FP32 engine on that same shape; the variant on pass 1's radix-16
+    # stages (the transform's first stages) and the FP32 engine on the default shape.
     precision = None
     precision_kw = {}
     if args.precision_templates > 0:
         wp = workloads.mixed_radix(count=args.precision_templates)
         precision = {"n": wp.n, "templates": len(wp.templates)}
         precision_kw["split_fp16_mfma"] = dict(precision=1)
-        for name in ("split_fp16_mfma", "fp32_same_shape", "fp32"):
+        precision_kw["split_fp16_mfma_pass1"] = dict(precision=2)
+        for name in ("split_fp16_mfma", "fp32_same_shape", "split_fp16_mfma_pass1", "fp32"):
             kw = precision_kw.get(name, {})
             with brp.GPU(wp.templates[0], wp.x, 256, 0, len(wp.templates), packed=True, engine=brp.ENGINE_FUSED,
                          graph=not args.no_graph, group=args.group, **kw) as gq:
@@ -398,10 +400,10 @@ def main() -> int:
                        "definition": "|P_gpu - P_fp64| / max(P_fp64, mean P_fp64), bank template 0"}
         log(f"[bench] power error vs FP64 oracle {power_error}")
         if precision is not None:
-            # The same error for the first template of the precision leg, both paths.
+            # The same error for the first template of the precision leg, every path.
             p0 = wp.templates[0]
             p64q = oracle_lib.reference(wp.x, p0)[3]
-            for name in ("split_fp16_mfma", "fp32_same_shape", "fp32"):
+            for name in ("split_fp16_mfma", "fp32_same_shape", "split_fp16_mfma_pass1", "fp32"):
                 with brp.GPU(p0, wp.x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, snapshot=True,
                              **precision_kw.get(name, {})) as gs:
                     gs.set_template(p0)

@@ -69,7 +69,7 @@ struct FusedArgs {
     uint32_t fp64_phase = 0;
     uint32_t fma = 0;                // flag 2 arithmetic (uniform over a handle)
     uint32_t nonneg = 0;             // every template of the group has omega, psi, dt >= 0
-    uint32_t precision = 0;          // 1: pass-2 radix-16 stages as split-FP16 MFMA GEMMs
+    uint32_t precision = 0;          // split-FP16 MFMA GEMMs: bit 0 pass-2, bit 1 pass-1 radix-16 stages
     uint32_t power_bins = 0;         // pass 2 writes power bins [0, power_bins) only (0: all M)
     // snapshot (per batch slot, or null)
     float* snap_resampled = nullptr;  // N floats
@@ -85,8 +85,9 @@ struct FusedArgs {
 // column pairs per block s2 and threads per block t1, t2 (each 0 = the first match).
 bool fused_supported(uint32_t n, uint32_t c, uint32_t s1, uint32_t s2, FusedShape* shape, uint32_t t1 = 0,
                      uint32_t t2 = 0);
-// Whether the shape's pass-2 plan starts with radix-16 stages (the MFMA variant's stages).
-bool fused_mfma_capable(const FusedShape& s);
+// Whether the shape has the radix-16 stages the precision variant runs as MFMA GEMMs: for
+// bit 0 a pass-2 plan starting with radix 16, for bit 1 a pass-1 shape compiled with them.
+bool fused_mfma_capable(const FusedShape& s, uint32_t precision);
 // The per-stage twiddle table of a fused FFT pass (length len, `elements` per thread): for
 // each stage i >= 1 of the radix plan (radix[i], ns[i] = product of the earlier radices) a
 // block at offset[i] whose entry (q - 1) ns[i] + k, q in [1, radix[i]), k < ns[i], holds

@@ -266,8 +266,9 @@ void check_options(const brp_params& p, const brp_options& o) {
     require(o.workgroup && o.workgroup <= 256 && o.batch && o.batch <= 4096 &&
                 (!o.tile || (pow2(o.tile) && o.tile >= 64 && o.tile <= 4096 && o.tile <= p.n)),
             "invalid workgroup, FFT tile, or batch");
-    require(o.precision <= 1, "unknown precision variant");
+    require(o.precision <= 3, "unknown precision variant");
     require(!o.precision || o.engine == BRP_ENGINE_FUSED, "the split-FP16 precision variant needs the fused engine");
+    require(!(o.precision & 2u) || !o.fp64_phase, "the split-FP16 pass-1 variant runs the FP32 index rule only");
 }
 
 void check_template(const brp_params& base, const brp_params& next) {
@@ -363,7 +364,8 @@ void add_fused_series(brp_hip& h, const brp_params& p, float* x) {
     s.n = p.n;
     s.unpadded = p.unpadded;
     // The first compiled shape of this length that fits the overrides (and, for the
-    // split-FP16 variant, whose pass-2 plan starts with radix 16); added series fall back to
+    // split-FP16 variant, that has the radix-16 stages it runs on the matrix cores); added
+    // series fall back to
     // ignoring the overrides.
     const auto pick = [&](bool overrides) {
         for (const auto& cand : brp_engine::fused_shapes(p.n)) {
@@ -373,7 +375,7 @@ void add_fused_series(brp_hip& h, const brp_params& p, float* x) {
                               (h.opt.split_t1 && cand.t1 != h.opt.split_t1) ||
                               (h.opt.split_t2 && cand.t2 != h.opt.split_t2)))
                 continue;
-            if (h.opt.precision && !brp_engine::fused_mfma_capable(cand)) continue;
+            if (h.opt.precision && !brp_engine::fused_mfma_capable(cand, h.opt.precision)) continue;
             s.shape = cand;
             return true;
         }
@@ -381,7 +383,7 @@ void add_fused_series(brp_hip& h, const brp_params& p, float* x) {
     };
     if (!pick(true))
         require(!h.series.empty() && pick(false),
-                h.opt.precision ? "split-FP16 variant: no shape of this length has radix-16 pass-2 stages"
+                h.opt.precision ? "split-FP16 variant: no shape of this length has the radix-16 stages it asks for"
                                 : "fused engine: unsupported series length (N = 2^13..2^23 or 3*2^15..3*2^22) or split");
     const auto& sh = s.shape;
     auto& a = s.tables;

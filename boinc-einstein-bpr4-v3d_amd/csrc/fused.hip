@@ -391,9 +391,11 @@ __device__ __forceinline__ void resample_pairs(cf* v, double& acc, bool& bad, co
 // SNAP: also write the resampled samples and their source indices (validation runs).
 // F64: orbital offset from the FP64 sine (a reported variant; the FP32 LUT rule is the
 // parity mode).
+// MF: the row FFT's radix-16 stages (the first stages of the transform) as split-FP16 MFMA
+// GEMMs (the precision variant; compiled for the shapes pass1_mfma_instance names).
 // At least 4 waves per SIMD (128 VGPRs): blocks of 512 threads then run two per CU; with
 // eight elements per thread, 6 waves per SIMD (80 VGPRs), three 512-thread blocks per CU.
-template <int C, int S, int T, bool SNAP, bool F64, bool NN, bool FM>
+template <int C, int S, int T, bool SNAP, bool F64, bool NN, bool FM, bool MF>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 8 && !F64 ? 6 : 4))) void k_pass1(
     FusedArgs a, FusedShape sh) {
     constexpr int E = C * S / T;
@@ -427,7 +429,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(C * S / T <= 
     // Keep the FFT's table reads out of the gather: hoisted there, their registers would
     // sit beside the gather's loads in flight.
     __builtin_amdgcn_sched_barrier(0);
-    fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
+    if constexpr (MF) {
+        const Dft16Mfma mf = dft16_mfma_setup();
+        fft_stages<C, S, T, 0, true>(v, lds, a.tw_c, t, &mf);
+    } else {
+        fft_stages<C, S, T, 0>(v, lds, a.tw_c, t);
+    }
 
     // Y[r][k2] = v * W_M^{r k2} for k2 = j + q C/RL: W_M^{r j} times
     // W_M^{r q C/RL} = W_{R RL}^{r q} (r q < R RL, read from the W_{16 R} table), all of a
@@ -992,6 +999,12 @@ __global__ __launch_bounds__(256) void brp_copy(const uint4* __restrict__ src, u
     for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
 }
 
+// Pass-1 shapes compiled with the split-FP16 MFMA row FFT (the 2048- and 4096-point rows the
+// 2^k and 3 * 2^k lengths use by default; both plans start with two radix-16 stages).
+constexpr bool pass1_mfma_instance(int c, int s, int t) {
+    return (c == 2048 && s == 2 && t == 256) || (c == 4096 && s == 2 && t == 512);
+}
+
 template <int C, int S, int T>
 void launch1(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
     const dim3 grid(sh.blocks1 * a.count), block(T);
@@ -999,23 +1012,32 @@ void launch1(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
     // Index form per group (the same in validation runs with snapshots); FP64 phase runs
     // take the general form.
     if (a.fp64_phase) {
-        if (snap) k_pass1<C, S, T, true, true, false, false><<<grid, block, 0, st>>>(a, sh);
-        else k_pass1<C, S, T, false, true, false, false><<<grid, block, 0, st>>>(a, sh);
+        if (snap) k_pass1<C, S, T, true, true, false, false, false><<<grid, block, 0, st>>>(a, sh);
+        else k_pass1<C, S, T, false, true, false, false, false><<<grid, block, 0, st>>>(a, sh);
         return;
     }
-    auto go = [&]<bool NN, bool FM>() {
-        if (snap) k_pass1<C, S, T, true, false, NN, FM><<<grid, block, 0, st>>>(a, sh);
-        else k_pass1<C, S, T, false, false, NN, FM><<<grid, block, 0, st>>>(a, sh);
+    auto go = [&]<bool NN, bool FM, bool MF>() {
+        if (snap) k_pass1<C, S, T, true, false, NN, FM, MF><<<grid, block, 0, st>>>(a, sh);
+        else k_pass1<C, S, T, false, false, NN, FM, MF><<<grid, block, 0, st>>>(a, sh);
     };
-    if (a.nonneg && a.fma) go.template operator()<true, true>();
-    else if (a.nonneg) go.template operator()<true, false>();
-    else if (a.fma) go.template operator()<false, true>();
-    else go.template operator()<false, false>();
+    auto form = [&]<bool MF>() {
+        if (a.nonneg && a.fma) go.template operator()<true, true, MF>();
+        else if (a.nonneg) go.template operator()<true, false, MF>();
+        else if (a.fma) go.template operator()<false, true, MF>();
+        else go.template operator()<false, false, MF>();
+    };
+    if constexpr (pass1_mfma_instance(C, S, T)) {
+        if (a.precision & 2u) {
+            form.template operator()<true>();
+            return;
+        }
+    }
+    form.template operator()<false>();
 }
 template <int R, int S, int T>
 void launch2(hipStream_t st, const FusedShape& sh, const FusedArgs& a) {
     if constexpr (plan_for(R, R * 2 * S / T).r[0] == 16) {
-        if (a.precision) {
+        if (a.precision & 1u) {
             if (a.snap_X) k_pass2<R, S, T, true, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
             else k_pass2<R, S, T, false, true><<<sh.blocks2 * a.count, T, 0, st>>>(a, sh);
             return;
@@ -1114,7 +1136,10 @@ constexpr ShapeRow kShapes[] = {
 
 namespace brp_engine {
 
-bool fused_mfma_capable(const FusedShape& s) { return plan_for(int(s.r), int(2 * s.r * s.s2 / s.t2)).r[0] == 16; }
+bool fused_mfma_capable(const FusedShape& s, uint32_t precision) {
+    const bool pass2 = plan_for(int(s.r), int(2 * s.r * s.s2 / s.t2)).r[0] == 16;
+    return (!(precision & 1u) || pass2) && (!(precision & 2u) || pass1_mfma_instance(int(s.c), int(s.s1), int(s.t1)));
+}
 
 StageTable fft_stage_table(uint32_t len, uint32_t elements) {
     const Radices pl = plan_for(int(len), int(elements));

@@ -78,7 +78,8 @@ typedef struct brp_options {
     uint32_t fp64_phase;   /* evaluate the orbital offset in FP64 (variant; indices may
                               differ from the FP32 LUT rule, which is the parity mode) */
     uint32_t split_c;      /* fused engine: pass-1 FFT length C (0 = default table) */
-    uint32_t precision;    /* 0 = FP32 FFT; 1 = split-FP16 MFMA first radix-16 stages */
+    uint32_t precision;    /* 0 = FP32 FFT; split-FP16 MFMA radix-16 stages: 1 = pass 2's,
+                              2 = pass 1's (the transform's first stages), 3 = both */
     uint32_t group;        /* fused engine: templates per launch (0 = 8) */
     uint32_t split_s1;     /* fused engine: pass-1 rows per block (0 = default for split_c) */
     uint32_t split_s2;     /* fused engine: pass-2 column pairs per block (0 = default) */

@@ -54,6 +54,13 @@ def test_create_validates_options_before_touching_the_device(brp):
         brp.GPU(p, x[:-1], 256, 0, 1)
     with pytest.raises(RuntimeError, match="invalid workgroup"):
         brp.GPU(p, x, 512, 0, 1)
+    # The precision variant: a bit mask of the passes whose radix-16 stages run on MFMA.
+    with pytest.raises(RuntimeError, match="unknown precision variant"):
+        brp.GPU(p, x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, precision=4)
+    with pytest.raises(RuntimeError, match="needs the fused engine"):
+        brp.GPU(p, x, 256, 0, 1, packed=True, precision=2)
+    with pytest.raises(RuntimeError, match="FP32 index rule only"):
+        brp.GPU(p, x, 256, 0, 1, packed=True, engine=brp.ENGINE_FUSED, precision=2, fp64_phase=True)
 
 
 def test_create_without_device_fails_cleanly(brp):
