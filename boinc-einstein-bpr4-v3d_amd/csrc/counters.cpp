@@ -17,6 +17,7 @@
 #include <rocprofiler-sdk/fwd.h>
 #include <rocprofiler-sdk/registration.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdlib>
@@ -44,6 +45,7 @@ struct Api {
 
 struct State {
     std::mutex mu;
+    std::mutex collect_mu;  // one collection at a time
     Api api;
     std::string reason = "in-process counters need BRP_HIP_COUNTERS=1 in the environment when the library loads";
     bool initialized = false;  // the tool's contexts exist
@@ -52,6 +54,7 @@ struct State {
     std::unordered_map<uint64_t, std::string> counter_names;                // counter id -> name
     std::map<uint64_t, std::map<std::string, uint64_t>> agent_counters;     // agent -> name -> id
     std::map<std::pair<uint64_t, std::string>, uint64_t> configs;           // (agent, set) -> config
+    std::map<std::pair<uint64_t, std::string>, std::vector<std::string>> config_missing;  // its absent names
     std::atomic<bool> collecting{false};
     std::vector<std::string> wanted;
     std::string wanted_key;
@@ -100,11 +103,11 @@ void on_dispatch(rocprofiler_dispatch_counting_service_data_t data, rocprofiler_
     if (it == s.configs.end()) {
         const auto& table = agent_counters(s, agent);
         std::vector<rocprofiler_counter_id_t> ids;
-        s.missing.clear();
+        auto& absent = s.config_missing[key];
         for (const auto& name : s.wanted) {
             auto c = table.find(name);
             if (c == table.end()) {
-                s.missing.push_back(name);
+                absent.push_back(name);
                 continue;
             }
             rocprofiler_counter_id_t id{};
@@ -116,6 +119,8 @@ void on_dispatch(rocprofiler_dispatch_counting_service_data_t data, rocprofiler_
             cfg.handle = 0;
         it = s.configs.emplace(key, cfg.handle).first;
     }
+    for (const auto& name : s.config_missing[key])
+        if (std::find(s.missing.begin(), s.missing.end(), name) == s.missing.end()) s.missing.push_back(name);
     if (!it->second) return;
     config->handle = it->second;
     Dispatch d;
@@ -234,6 +239,7 @@ std::string unavailable_reason() {
 std::vector<Dispatch> collect(const std::vector<std::string>& counters, const std::function<void()>& body,
                               std::vector<std::string>* missing) {
     auto& s = state();
+    std::lock_guard<std::mutex> one(s.collect_mu);
     {
         std::lock_guard<std::mutex> lock(s.mu);
         if (!s.initialized) throw std::runtime_error(s.reason);

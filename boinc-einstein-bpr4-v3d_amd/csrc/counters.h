@@ -29,9 +29,10 @@ struct Dispatch {
 };
 
 // Runs `body` (which enqueues work and waits for it) with `counters` collected on every kernel
-// dispatch it issues; returns the dispatches in issue order.  Counters the device does not
-// support are left out (`missing` lists them).  Throws std::runtime_error when unavailable or
-// when the records do not arrive.
+// dispatch the process issues meanwhile (other threads' included); returns the dispatches in
+// issue order.  Counters the device does not support are left out (`missing` lists them).
+// Calls from several threads run one after another.  Throws std::runtime_error when
+// unavailable or when the records do not arrive.
 std::vector<Dispatch> collect(const std::vector<std::string>& counters, const std::function<void()>& body,
                               std::vector<std::string>* missing = nullptr);
 

@@ -1088,8 +1088,14 @@ int brp_hip_autotune(const brp_params* templates, size_t count, const float* inp
         require(templates && count && input && base && best, "null argument");
         require(reps > 0 && reps <= 1000, "invalid repetitions");
         const uint32_t n = templates[0].n;
-        const std::vector<brp_engine::FusedShape> shapes = brp_engine::fused_shapes(n);
-        require(!shapes.empty(), "fused engine: unsupported series length");
+        // The compiled shapes of the length; with the precision variant, those that have the
+        // radix-16 stages it runs on the matrix cores.
+        std::vector<brp_engine::FusedShape> shapes;
+        for (const auto& s : brp_engine::fused_shapes(n))
+            if (!base->precision || brp_engine::fused_mfma_capable(s, base->precision)) shapes.push_back(s);
+        require(!shapes.empty(), base->precision ? "split-FP16 variant: no shape of this length has the radix-16 "
+                                                   "stages it asks for"
+                                                 : "fused engine: unsupported series length");
         std::ostringstream log;
         double best_us = 1e300;
         *best = *base;

@@ -252,6 +252,23 @@ def test_autotune_picks_a_correct_configuration(brp, wl):
         assert not cc["mismatches"], cc
 
 
+def test_autotune_with_precision_variant(brp, wl):
+    """With the split-FP16 variant the tuner times only the shapes that have the radix-16
+    stages it asks for, and its pick runs with the variant."""
+    w = wl.bank(count=4, n=1 << 20)
+    say("[gpu] autotune N=2^20, 4 templates, precision=1 and precision=2")
+    for prec in (1, 2):
+        best, report = brp.autotune(w.templates, w.x, {"precision": prec, "lanes": 1}, reps=1)
+        say(prec, best, report)
+        assert report and all(r["us_per_template"] > 0 for r in report)
+        if prec & 2:  # pass-1 shapes compiled with the variant: 2 rows of 2048 in 256 threads
+            assert {(r["C"], r["S1"], r["T1"]) for r in report} == {(2048, 2, 256)}
+        with brp.GPU(w.templates[0], w.x, 256, 0, 4, packed=True, engine=brp.ENGINE_FUSED, precision=prec,
+                     **best) as g:
+            g.set_templates(w.templates)
+            g.run()
+
+
 def test_full_size_mixed_radix(brp, oracle, wl):
     w = wl.mixed_radix(count=3)
     say("[gpu] fused N=3*2^20, 3 templates in one launch")
