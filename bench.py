@@ -455,6 +455,15 @@ def main() -> int:
                               "fft_flop_per_template": round(5 * (n // 2) * np.log2(n // 2)),
                               "fft_tflops": round(5 * (n // 2) * np.log2(n // 2) * templates_per_s / 1e12, 3)},
         "kernel_us_per_template": {k: round(v * 1e3 / max(timed_templates, 1), 3) for k, v in kms.items()},
+        # Every fused kernel against the same peak, and against the streaming-copy rate this
+        # run measured (the HBM rate a plain copy reaches on this device).
+        "kernels_roofline": {k: {"algorithmic_bytes_per_template": kb[k],
+                                 "achieved_GBps": round(kb[k] * timed_templates / (v * 1e-3) / 1e9, 2) if v > 0 else None,
+                                 "frac": round(kb[k] * timed_templates / (v * 1e-3) / HBM_PEAK, 4) if v > 0 else None,
+                                 "frac_of_copy": round(kb[k] * timed_templates / (v * 1e-3) / (copy_gbps * 1e9), 4)
+                                 if v > 0 and copy_gbps else None}
+                             for k, v in kms.items()},
+        "pipeline_frac_of_copy": round(pipeline_achieved / (copy_gbps * 1e9), 4) if copy_gbps else None,
         "kernel_timing": {"timed_launches": launches, "timed_templates": timed_templates, "every_nth_group": stride},
         "copy_GBps": round(copy_gbps, 1),
         "cpu_baseline": cpu,
